@@ -1,0 +1,295 @@
+"""ctypes binding of the femgpu C-ABI (include/femgpu.h).
+
+The shared library ``libfemgpu.so`` is built in-tree next to this file
+(``cudafemproject_amd.build.build_library``).  Loading fails loudly when it is
+missing: there is no CPU fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_NAME = "libfemgpu.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+# Status codes (femgpu.h)
+OK = 0
+E_ARG, E_HIP, E_NONFINITE, E_MAXIT, E_BREAKDOWN, E_BAD_MESH, E_STATE, E_NOMEM, E_UNSUPPORTED = (
+    -1, -2, -3, -4, -5, -6, -7, -8, -9)
+STATUS_NAMES = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_NONFINITE", -4: "E_MAXIT",
+                -5: "E_BREAKDOWN", -6: "E_BAD_MESH", -7: "E_STATE", -8: "E_NOMEM",
+                -9: "E_UNSUPPORTED"}
+
+FP64, FP32, MIXED = 0, 1, 2
+TOL_REF_GAMMA, TOL_REL_RES = 0, 1
+LOADS_REFERENCE, LOADS_PHYSICAL = 0, 1
+MASS_REFERENCE, MASS_PHYSICAL = 0, 1
+WAVELET_NONE, WAVELET_RICKER, WAVELET_BERLAGE = 0, 1, 2
+IMPLICIT_REFERENCE, IMPLICIT_PHYSICAL = 0, 1
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("tol_kind", C.c_int), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("inner_tol", C.c_double), ("check_every", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("outer_iterations", C.c_int),
+                ("rel_res", C.c_double), ("gamma_ratio", C.c_double), ("solve_ms", C.c_double)]
+
+
+class DynOpts(C.Structure):
+    _fields_ = [("rho", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("dt", C.c_double), ("endtime", C.c_double), ("wavelet_type", C.c_int),
+                ("wavelet_freq", C.c_double), ("wavelet_timeshift", C.c_double),
+                ("wavelet_ampl", C.c_double), ("precision", C.c_int),
+                ("mass_convention", C.c_int), ("implicit_mode", C.c_int),
+                ("pcg_eps", C.c_double), ("steps_per_graph", C.c_int)]
+
+
+class DynStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("steps", C.c_int), ("pcg_iterations", C.c_int),
+                ("run_ms", C.c_double)]
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+
+# name -> (restype, argtypes); the list of exported symbols of include/femgpu.h
+SIGNATURES = {
+    "femgpu_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "femgpu_destroy": (None, [_P]),
+    "femgpu_last_error": (C.c_char_p, [_P]),
+    "femgpu_set_stream": (C.c_int, [_P, _P]),
+    "femgpu_sync": (C.c_int, [_P]),
+    "femgpu_set_mesh": (C.c_int, [_P, C.c_int, C.c_int64, _D, C.c_int64, _I]),
+    "femgpu_set_material": (C.c_int, [_P, C.c_double, C.c_double, _D]),
+    "femgpu_set_constraints": (C.c_int, [_P, C.c_int64, _I, _I]),
+    "femgpu_set_pressure_faces": (C.c_int, [_P, C.c_int64, _I, _I, _D, _D]),
+    "femgpu_set_nodal_loads": (C.c_int, [_P, C.c_int64, _I, _D]),
+    "femgpu_set_load_convention": (C.c_int, [_P, C.c_int]),
+    "femgpu_set_extra_rhs": (C.c_int, [_P, _D]),
+    "femgpu_matvec": (C.c_int, [_P, C.c_int, _D, _D]),
+    "femgpu_diag": (C.c_int, [_P, _D]),
+    "femgpu_rhs": (C.c_int, [_P, _D]),
+    "femgpu_matvec_repeat": (C.c_int, [_P, C.c_int, C.c_int]),
+    "femgpu_default_solve_opts": (C.c_int, [C.POINTER(SolveOpts)]),
+    "femgpu_solve_static": (C.c_int, [_P, C.POINTER(SolveOpts), _D, C.POINTER(Stats)]),
+    "femgpu_stress": (C.c_int, [_P, _D, _D]),
+    "femgpu_element_energy": (C.c_int, [_P, _D, _D]),
+    "femgpu_default_dyn_opts": (C.c_int, [C.POINTER(DynOpts)]),
+    "femgpu_dyn_run": (C.c_int, [_P, C.POINTER(DynOpts), _D, C.POINTER(DynStats)]),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libfemgpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950); "
+            "there is no CPU fallback")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class FemGpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"femgpu error {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(_I)
+
+
+class Context:
+    """One femgpu context (one device, one stream)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.femgpu_create(C.byref(h), device)
+        if st != OK:
+            raise FemGpuError(st, "femgpu_create failed (no HIP device?)")
+        self.h = h
+        self.dim = 0
+        self.n_nodes = 0
+        self.n_elems = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.femgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st: int, what: str):
+        if st != OK:
+            msg = self.lib.femgpu_last_error(self.h)
+            raise FemGpuError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    # --- problem -----------------------------------------------------------
+    def set_mesh(self, nodes: np.ndarray, elems: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        dim = nodes.shape[1]
+        self._check(self.lib.femgpu_set_mesh(self.h, dim, nodes.shape[0], _dp(nodes),
+                                             elems.shape[0], _ip(elems)), "set_mesh")
+        self.dim, self.n_nodes, self.n_elems = dim, nodes.shape[0], elems.shape[0]
+
+    def set_material(self, E: float, nu: float, elem_scale: np.ndarray | None = None):
+        if elem_scale is not None:
+            elem_scale = np.ascontiguousarray(elem_scale, dtype=np.float64)
+            assert elem_scale.shape == (self.n_elems,)
+            p = _dp(elem_scale)
+        else:
+            p = None
+        self._check(self.lib.femgpu_set_material(self.h, E, nu, p), "set_material")
+
+    def set_constraints(self, nodes: np.ndarray, codes: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        codes = np.ascontiguousarray(codes, dtype=np.int32)
+        self._check(self.lib.femgpu_set_constraints(self.h, len(nodes), _ip(nodes), _ip(codes)),
+                    "set_constraints")
+
+    def set_pressure_faces(self, face_nodes, adj_elem, normals, pressures):
+        fn = np.ascontiguousarray(face_nodes, dtype=np.int32)
+        ae = np.ascontiguousarray(adj_elem, dtype=np.int32)
+        nr = np.ascontiguousarray(normals, dtype=np.float64)
+        pr = np.ascontiguousarray(pressures, dtype=np.float64)
+        self._check(self.lib.femgpu_set_pressure_faces(self.h, len(ae), _ip(fn), _ip(ae), _dp(nr),
+                                                       _dp(pr)), "set_pressure_faces")
+
+    def set_nodal_loads(self, nodes, vecs):
+        nd = np.ascontiguousarray(nodes, dtype=np.int32)
+        vc = np.ascontiguousarray(vecs, dtype=np.float64)
+        self._check(self.lib.femgpu_set_nodal_loads(self.h, len(nd), _ip(nd), _dp(vc)),
+                    "set_nodal_loads")
+
+    def set_load_convention(self, conv: int):
+        self._check(self.lib.femgpu_set_load_convention(self.h, conv), "set_load_convention")
+
+    def set_extra_rhs(self, f: np.ndarray | None):
+        if f is None:
+            self._check(self.lib.femgpu_set_extra_rhs(self.h, None), "set_extra_rhs")
+            return
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        assert f.size == self.n_nodes * self.dim
+        self._check(self.lib.femgpu_set_extra_rhs(self.h, _dp(f)), "set_extra_rhs")
+
+    def set_problem(self, pb):
+        """Load a `Problem` (cudafemproject_amd.problem) into the context."""
+        self.set_mesh(pb.nodes, pb.elems)
+        self.set_material(pb.E, pb.nu, pb.elem_scale)
+        if len(pb.constr_node):
+            self.set_constraints(pb.constr_node, pb.constr_code)
+        if len(pb.face_adj):
+            self.set_pressure_faces(pb.face_nodes, pb.face_adj, pb.face_normal, pb.face_p)
+        if len(pb.load_node):
+            self.set_nodal_loads(pb.load_node, pb.load_vec)
+        self.set_load_convention(pb.convention)
+        if pb.extra_rhs is not None:
+            self.set_extra_rhs(pb.extra_rhs)
+
+    # --- operators -----------------------------------------------------------
+    def matvec(self, x: np.ndarray, precision: int = FP64) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.n_nodes, self.dim)
+        y = np.zeros_like(x)
+        self._check(self.lib.femgpu_matvec(self.h, precision, _dp(x), _dp(y)), "matvec")
+        return y
+
+    def diag(self) -> np.ndarray:
+        d = np.zeros((self.n_nodes, self.dim))
+        self._check(self.lib.femgpu_diag(self.h, _dp(d)), "diag")
+        return d
+
+    def rhs(self) -> np.ndarray:
+        f = np.zeros((self.n_nodes, self.dim))
+        self._check(self.lib.femgpu_rhs(self.h, _dp(f)), "rhs")
+        return f
+
+    def matvec_repeat(self, precision: int, reps: int):
+        self._check(self.lib.femgpu_matvec_repeat(self.h, precision, reps), "matvec_repeat")
+
+    def set_stream(self, stream_ptr: int | None):
+        self._check(self.lib.femgpu_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None),
+                    "set_stream")
+
+    def sync(self):
+        self._check(self.lib.femgpu_sync(self.h), "sync")
+
+    # --- solves ----------------------------------------------------------------
+    def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
+              max_iter: int = 0, inner_tol: float = 1e-4, check_every: int = 32,
+              raise_on_error: bool = True):
+        o = SolveOpts()
+        self.lib.femgpu_default_solve_opts(C.byref(o))
+        o.precision, o.tol, o.tol_kind, o.max_iter = precision, tol, tol_kind, max_iter
+        o.inner_tol, o.check_every = inner_tol, check_every
+        u = np.zeros((self.n_nodes, self.dim))
+        s = Stats()
+        st = self.lib.femgpu_solve_static(self.h, C.byref(o), _dp(u), C.byref(s))
+        if raise_on_error:
+            self._check(st, "solve_static")
+        return u, {"status": s.status, "iterations": s.iterations,
+                   "outer_iterations": s.outer_iterations, "rel_res": s.rel_res,
+                   "gamma_ratio": s.gamma_ratio, "solve_ms": s.solve_ms}
+
+    def stress(self, u: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nv = 3 * (self.dim - 1)
+        s = np.zeros((self.n_elems, nv))
+        self._check(self.lib.femgpu_stress(self.h, _dp(u), _dp(s)), "stress")
+        return s
+
+    def element_energy(self, u: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        e = np.zeros(self.n_elems)
+        self._check(self.lib.femgpu_element_energy(self.h, _dp(u), _dp(e)), "element_energy")
+        return e
+
+    def dyn_run(self, rho=2400.0, beta1=0.5, beta2=0.0, dt=1e-3, endtime=0.1,
+                wavelet_type=WAVELET_NONE, freq=15.0, timeshift=0.0, ampl=1.0,
+                precision=FP64, mass_convention=MASS_REFERENCE,
+                implicit_mode=IMPLICIT_REFERENCE, pcg_eps=1e-4, steps_per_graph=64):
+        o = DynOpts()
+        self.lib.femgpu_default_dyn_opts(C.byref(o))
+        o.rho, o.beta1, o.beta2, o.dt, o.endtime = rho, beta1, beta2, dt, endtime
+        o.wavelet_type, o.wavelet_freq, o.wavelet_timeshift, o.wavelet_ampl = (
+            wavelet_type, freq, timeshift, ampl)
+        o.precision, o.mass_convention, o.implicit_mode = precision, mass_convention, implicit_mode
+        o.pcg_eps, o.steps_per_graph = pcg_eps, steps_per_graph
+        u = np.zeros((self.n_nodes, self.dim))
+        s = DynStats()
+        self._check(self.lib.femgpu_dyn_run(self.h, C.byref(o), _dp(u), C.byref(s)), "dyn_run")
+        return u, {"status": s.status, "steps": s.steps, "pcg_iterations": s.pcg_iterations,
+                   "run_ms": s.run_ms}

@@ -1,0 +1,82 @@
+"""In-tree build of libfemgpu.so (hipcc, gfx950) and of the FEMproject-compatible
+host executable ``fem_main``.
+
+The HIP code is compiled for gfx950 only, with packed-FP32 instructions
+disabled (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libfemgpu.so")
+
+ARCH = os.environ.get("FEMGPU_ARCH", "gfx950")
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function"]
+
+LIB_SOURCES = ["kernels.hip", "femgpu_api.cpp", "mesh_plan.cpp"]
+
+
+def _hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ...")
+    return r
+
+
+def build_library(out: str = LIB) -> str:
+    """Compile libfemgpu.so (always recompiles)."""
+    hipcc = _hipcc()
+    objs = []
+    for src in LIB_SOURCES:
+        o = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        _run([hipcc, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+              os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    tmp = out + ".tmp"
+    _run([hipcc, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_fem_main() -> str | None:
+    """Compile the FEMproject-compatible driver executable (femproject/),
+    linked against libfemgpu.so with an rpath to this directory."""
+    src_dir = os.path.join(PKG, "femproject")
+    if not os.path.isdir(src_dir):
+        return None
+    srcs = sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir) if f.endswith(".cpp"))
+    if not srcs:
+        return None
+    out = os.path.join(PKG, "fem_main")
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{src_dir}", "-o", out, *srcs,
+          f"-L{PKG}", "-lfemgpu", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
+def build_all():
+    build_library()
+    build_fem_main()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB)

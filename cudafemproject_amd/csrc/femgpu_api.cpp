@@ -1,0 +1,1224 @@
+// femgpu_api.cpp -- C-ABI implementation (see include/femgpu.h).
+//
+// Host side of the solve path: keeps the user's problem (user node/element
+// order), the planned device mesh (internal order, femgpu_internal.h), the
+// device vectors, and drives the HIP kernels of kernels.hip on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "femgpu.h"
+#include "femgpu_internal.h"
+#include "kernels.h"
+
+using femgpu::CgScalars;
+using femgpu::DevMesh;
+using femgpu::MeshPlan;
+using femgpu::OpShift;
+
+namespace {
+
+constexpr int NODE_PART = 2 * 2048;  // per-block partials of node kernels (2 arrays)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+// Wavelet value exactly as ElementsData::updateWavelet (fem_utils.cpp:53-82):
+// float intermediates, M_PI products formed in double.
+float wavelet_value(float t, int type, float freq, float timeshift, float ampl) {
+  const double PI = 3.14159265358979323846;
+  if (t < timeshift) return 0.f;
+  t = t - timeshift;
+  if (type == FEMGPU_WAVELET_RICKER) {
+    float tmp = float(PI * double(freq) * double(t) - PI);
+    return ampl * (1.f - 2.f * tmp * tmp) * std::exp(-1.f * tmp * tmp);
+  }
+  if (type == FEMGPU_WAVELET_BERLAGE) {
+    float w0 = float(2 * PI * double(freq));
+    float w1 = w0 / std::sqrt(3.f);
+    return ampl * w1 * w1 / 4.f * std::exp(-1.f * w1 * t) *
+           (std::sin(w0 * t) * (1.f / (w1 * w1 * w1) + t / (w1 * w1) - t * t / w1) -
+            std::cos(w0 * t) * std::sqrt(3.f) * (t * t / w1 + t / (w1 * w1)));
+  }
+  return 0.f;
+}
+
+}  // namespace
+
+struct femgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+
+  // problem, user order
+  int dim = 0;
+  int64_t nn = 0, ne = 0;
+  std::vector<double> nodes;
+  std::vector<int32_t> elems;
+  double E = 2e7, nu = 0.3;
+  std::vector<double> escale;
+  std::vector<int32_t> cnode, ccode;
+  std::vector<int32_t> fnodes, fadj;
+  std::vector<double> fnormal, fp;
+  std::vector<int32_t> lnode;
+  std::vector<double> lvec;
+  std::vector<double> extra_rhs;
+  int conv = FEMGPU_LOADS_REFERENCE;
+  int emax = 0;  // elements per patch (0: default)
+
+  bool have_mesh = false;
+  MeshPlan plan;
+  DevMesh dm;
+  std::map<std::string, DevBuf> bufs;
+  std::vector<uint8_t> fix_user;  // per user node
+  std::vector<int32_t> ctype;     // per user node (last line wins)
+
+  bool dirty_mat = true, dirty_fix = true, dirty_diag = true;
+
+  CgScalars *h_sc = nullptr;  // pinned
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // graph cache for PCG iteration blocks, keyed by everything baked into
+  // the captured launches (buffers, operator shift, block length)
+  std::map<std::string, hipGraphExec_t> graphs;
+  uint64_t buf_gen = 0;
+  uint64_t graph_gen = 0;
+};
+
+namespace {
+
+int fail(femgpu_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ctx, FEMGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void drop_graphs(femgpu_ctx *ctx) {
+  for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  ctx->graphs.clear();
+}
+
+template <typename T>
+int dbuf(femgpu_ctx *ctx, const std::string &name, size_t count, T **out) {
+  DevBuf &b = ctx->bufs[name];
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (b.bytes < bytes) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+      b.p = nullptr;
+      return fail(ctx, FEMGPU_E_NOMEM, "hipMalloc failed for " + name);
+    }
+    b.bytes = bytes;
+    ctx->buf_gen++;
+  }
+  *out = static_cast<T *>(b.p);
+  return FEMGPU_OK;
+}
+
+template <typename T>
+int upload(femgpu_ctx *ctx, const std::string &name, const std::vector<T> &h, T **out) {
+  int st = dbuf<T>(ctx, name, h.size(), out);
+  if (st) return st;
+  if (!h.empty())
+    HIPCHK(hipMemcpyAsync(*out, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return FEMGPU_OK;
+}
+
+// user node vector (nn x dim) -> internal order
+std::vector<double> to_internal(const femgpu_ctx *ctx, const double *u) {
+  const int d = ctx->dim;
+  std::vector<double> v(size_t(ctx->nn) * d);
+  for (int64_t i = 0; i < ctx->nn; ++i) {
+    const int64_t s = ctx->plan.node_perm[i];
+    for (int c = 0; c < d; ++c) v[size_t(i) * d + c] = u[size_t(s) * d + c];
+  }
+  return v;
+}
+
+void from_internal(const femgpu_ctx *ctx, const std::vector<double> &v, double *u) {
+  const int d = ctx->dim;
+  for (int64_t i = 0; i < ctx->nn; ++i) {
+    const int64_t s = ctx->plan.node_perm[i];
+    for (int c = 0; c < d; ++c) u[size_t(s) * d + c] = v[size_t(i) * d + c];
+  }
+}
+
+void lame(int dim, double E, double nu, double &lam, double &mu) {
+  mu = E / (2.0 * (1.0 + nu));
+  lam = dim == 3 ? E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)) : E * nu / (1.0 - nu * nu);
+}
+
+int ensure_material(femgpu_ctx *ctx) {
+  if (!ctx->dirty_mat) return FEMGPU_OK;
+  lame(ctx->dim, ctx->E, ctx->nu, ctx->dm.lam, ctx->dm.mu);
+  if (!ctx->escale.empty()) {
+    std::vector<double> s64(ctx->ne);
+    std::vector<float> s32(ctx->ne);
+    for (int64_t i = 0; i < ctx->ne; ++i) {
+      s64[i] = ctx->escale[ctx->plan.elem_perm[i]];
+      s32[i] = float(s64[i]);
+    }
+    double *p64;
+    float *p32;
+    int st = upload(ctx, "escale64", s64, &p64);
+    if (!st) st = upload(ctx, "escale32", s32, &p32);
+    if (st) return st;
+    ctx->dm.escale64 = p64;
+    ctx->dm.escale32 = p32;
+  } else {
+    ctx->dm.escale64 = nullptr;
+    ctx->dm.escale32 = nullptr;
+  }
+  ctx->dirty_mat = false;
+  ctx->dirty_diag = true;
+  drop_graphs(ctx);
+  return FEMGPU_OK;
+}
+
+int ensure_fix(femgpu_ctx *ctx) {
+  if (!ctx->dirty_fix) return FEMGPU_OK;
+  std::vector<uint8_t> fm(ctx->nn, 0);
+  for (int64_t i = 0; i < ctx->nn; ++i) fm[i] = ctx->fix_user[ctx->plan.node_perm[i]];
+  uint8_t *p;
+  int st = upload(ctx, "fixmask", fm, &p);
+  if (st) return st;
+  ctx->dm.fixmask = p;
+  ctx->dirty_fix = false;
+  return FEMGPU_OK;
+}
+
+// Assembled diagonal (FP64) and its inverse in FP64 and FP32.
+int ensure_diag(femgpu_ctx *ctx) {
+  int st = ensure_material(ctx);
+  if (!st) st = ensure_fix(ctx);
+  if (st) return st;
+  if (!ctx->dirty_diag) return FEMGPU_OK;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  double *diag, *inv, *part;
+  float *inv32;
+  if ((st = dbuf(ctx, "diag64", nd, &diag)) || (st = dbuf(ctx, "invD64", nd, &inv)) ||
+      (st = dbuf(ctx, "invD32", nd, &inv32)) ||
+      (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim, &part)))
+    return st;
+  femgpu::launch_fill<double>(diag, int64_t(nd), 0.0, ctx->stream);
+  femgpu::launch_diag<double>(ctx->dm, diag, part, ctx->stream);
+  femgpu::launch_inv_diag<double>(ctx->dm, diag, 1.0, 0.0, nullptr, inv, ctx->stream);
+  femgpu::launch_convert<double, float>(inv, inv32, int64_t(nd), ctx->stream);
+  HIPCHK(hipGetLastError());
+  ctx->dirty_diag = false;
+  return FEMGPU_OK;
+}
+
+// Global RHS in user order (FP64).  unit_pressure: evaluate every face with
+// pressure 1 (the time-dependent part of the dynamics RHS).
+void host_rhs(const femgpu_ctx *ctx, bool unit_pressure, std::vector<double> &f) {
+  const int d = ctx->dim, npe = d + 1, nde = d * npe;
+  f.assign(size_t(ctx->nn) * d, 0.0);
+  const int64_t nf = int64_t(ctx->fadj.size());
+  const int64_t nl = int64_t(ctx->lnode.size());
+  if (ctx->conv == FEMGPU_LOADS_REFERENCE) {
+    if (nl > 0) {
+      // calcLoads: loads gathered into every element slot, overwriting the
+      // pressures (cpu_fem_utils.cpp:588-596) -> assembled load x valence.
+      std::vector<double> loads(size_t(ctx->nn) * d, 0.0);
+      for (int64_t l = 0; l < nl; ++l)
+        for (int c = 0; c < d; ++c) loads[size_t(ctx->lnode[l]) * d + c] = ctx->lvec[l * d + c];
+      for (int64_t e = 0; e < ctx->ne; ++e)
+        for (int a = 0; a < npe; ++a) {
+          const int64_t n = ctx->elems[e * npe + a];
+          for (int c = 0; c < d; ++c) f[size_t(n) * d + c] += loads[size_t(n) * d + c];
+        }
+    } else if (nf > 0) {
+      // calcPressures (cpu_fem_utils.cpp:564-582): assignment per element slot.
+      std::unordered_map<int64_t, double> slot;
+      slot.reserve(size_t(nf) * nde);
+      std::vector<int64_t> order;
+      const double coeff = d == 2 ? -0.5 : -1.0 / 12.0;
+      for (int64_t fc = 0; fc < nf; ++fc) {
+        const int64_t el = ctx->fadj[fc];
+        const double *p0 = &ctx->nodes[size_t(ctx->fnodes[fc * d + 0]) * d];
+        const double *p1 = &ctx->nodes[size_t(ctx->fnodes[fc * d + 1]) * d];
+        double len;
+        if (d == 2) {
+          len = std::sqrt((p1[0] - p0[0]) * (p1[0] - p0[0]) + (p1[1] - p0[1]) * (p1[1] - p0[1]));
+        } else {
+          const double *p2 = &ctx->nodes[size_t(ctx->fnodes[fc * d + 2]) * d];
+          const double ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
+          const double bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
+          const double a1 = ay * bz - az * by, a2 = az * bx - ax * bz, a3 = ax * by - ay * bx;
+          len = 0.5 * (a1 * a1 + a2 * a2 + a3 * a3);
+        }
+        const double pr = unit_pressure ? 1.0 : ctx->fp[fc];
+        for (int i = 0; i < d; ++i) {
+          const int32_t node = ctx->fnodes[fc * d + i];
+          int loc = 0;
+          for (int a = 0; a < npe; ++a)
+            if (ctx->elems[el * npe + a] == node) {
+              loc = a;
+              break;
+            }
+          for (int j = 0; j < d; ++j) {
+            const int64_t key = el * nde + d * loc + j;
+            const double v =
+                (ctx->ctype[node] & j) ? 0.0 : coeff * pr * len * ctx->fnormal[fc * d + j];
+            if (!slot.count(key)) order.push_back(key);
+            slot[key] = v;
+          }
+        }
+      }
+      std::sort(order.begin(), order.end());
+      for (int64_t key : order) {
+        const int64_t el = key / nde, k = key % nde;
+        const int64_t n = ctx->elems[el * npe + k / d];
+        f[size_t(n) * d + k % d] += slot[key];
+      }
+    }
+  } else {
+    // physical: F_node = -p A / d n per face node, duplicates once, summed.
+    std::map<std::vector<int32_t>, int64_t> seen;
+    for (int64_t fc = 0; fc < nf; ++fc) {
+      std::vector<int32_t> key(ctx->fnodes.begin() + fc * d, ctx->fnodes.begin() + fc * d + d);
+      std::sort(key.begin(), key.end());
+      if (seen.count(key)) continue;
+      seen[key] = fc;
+      const double *p0 = &ctx->nodes[size_t(ctx->fnodes[fc * d + 0]) * d];
+      const double *p1 = &ctx->nodes[size_t(ctx->fnodes[fc * d + 1]) * d];
+      double A;
+      if (d == 2) {
+        A = std::sqrt((p1[0] - p0[0]) * (p1[0] - p0[0]) + (p1[1] - p0[1]) * (p1[1] - p0[1]));
+      } else {
+        const double *p2 = &ctx->nodes[size_t(ctx->fnodes[fc * d + 2]) * d];
+        const double ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
+        const double bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
+        const double a1 = ay * bz - az * by, a2 = az * bx - ax * bz, a3 = ax * by - ay * bx;
+        A = 0.5 * std::sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+      }
+      const double pr = unit_pressure ? 1.0 : ctx->fp[fc];
+      for (int i = 0; i < d; ++i) {
+        const int32_t node = ctx->fnodes[fc * d + i];
+        for (int j = 0; j < d; ++j) f[size_t(node) * d + j] += -pr * A / d * ctx->fnormal[fc * d + j];
+      }
+    }
+    for (int64_t l = 0; l < nl; ++l)
+      for (int c = 0; c < d; ++c) f[size_t(ctx->lnode[l]) * d + c] += ctx->lvec[l * d + c];
+    for (int64_t n = 0; n < ctx->nn; ++n)
+      for (int c = 0; c < d; ++c)
+        if ((ctx->fix_user[n] >> c) & 1) f[size_t(n) * d + c] = 0.0;
+  }
+  if (!unit_pressure && !ctx->extra_rhs.empty())
+    for (size_t i = 0; i < f.size(); ++i) f[i] += ctx->extra_rhs[i];
+  // nodes no element references carry no equation
+  for (int64_t i = ctx->plan.n_owned + ctx->plan.n_iface; i < ctx->nn; ++i)
+    for (int c = 0; c < d; ++c) f[size_t(ctx->plan.node_perm[i]) * d + c] = 0.0;
+}
+
+int poll(femgpu_ctx *ctx, const CgScalars *d_sc) {
+  HIPCHK(hipMemcpyAsync(ctx->h_sc, d_sc, sizeof(CgScalars), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return FEMGPU_OK;
+}
+
+// Workspace of one PCG precision.
+template <typename T>
+struct PcgWs {
+  T *x, *r, *z, *p, *q, *partial;
+  const T *invD;
+};
+
+template <typename T>
+int pcg_ws(femgpu_ctx *ctx, const char *tag, PcgWs<T> &w) {
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  std::string t(tag);
+  int st;
+  if ((st = dbuf(ctx, t + "x", nd, &w.x)) || (st = dbuf(ctx, t + "r", nd, &w.r)) ||
+      (st = dbuf(ctx, t + "z", nd, &w.z)) || (st = dbuf(ctx, t + "p", nd, &w.p)) ||
+      (st = dbuf(ctx, t + "q", nd, &w.q)) ||
+      (st = dbuf(ctx, t + "partial", size_t(ctx->plan.n_slots) * ctx->dim + 1, &w.partial)))
+    return st;
+  return FEMGPU_OK;
+}
+
+// Runs PCG iterations until the device-side stop flag is set.  r holds the
+// initial residual, x must be zero.  Iteration blocks of `block` steps are
+// captured once into a HIP graph and replayed.
+template <typename T>
+int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, int max_iter,
+            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code) {
+  double *part_patch, *part_node;
+  int st;
+  if ((st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)) ||
+      (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
+    return st;
+  const int64_t nd = ctx->nn * ctx->dim;
+  femgpu::launch_fill<T>(w.x, nd, T(0), ctx->stream);
+  femgpu::launch_fill<T>(w.p, nd, T(0), ctx->stream);
+  femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
+  femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.z, w.invD, sc, part_node, ctx->stream);
+  HIPCHK(hipGetLastError());
+  if (block < 1) block = 1;
+  // graph for `block` iterations on these buffers
+  if (ctx->graph_gen != ctx->buf_gen) {
+    drop_graphs(ctx);
+    ctx->graph_gen = ctx->buf_gen;
+  }
+  char keybuf[256];
+  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block,
+                (const void *)w.x, (const void *)w.invD, (const void *)sc, (const void *)sh.m,
+                (const void *)w.partial, (const void *)part_patch, double(sh.cK), double(sh.cM));
+  const std::string key(keybuf);
+  hipGraphExec_t exec = nullptr;
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end()) {
+    exec = it->second;
+  } else {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < block; ++k)
+      femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.z, w.p, w.q, w.partial, w.invD, sh, sc,
+                                 part_patch, part_node, ctx->stream);
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    ctx->graphs[key] = exec;
+  }
+  // Every replay either advances the device iteration counter or sets the
+  // stop flag; anything else means the kernels did not run.
+  int last_iter = -1;
+  const long max_replays = long(max_iter) / block + 2;
+  for (long rep = 0;; ++rep) {
+    HIPCHK(hipGraphLaunch(exec, ctx->stream));
+    if ((st = poll(ctx, sc))) return st;
+    if (ctx->h_sc->done) break;
+    if (ctx->h_sc->iter <= last_iter || rep > max_replays)
+      return fail(ctx, FEMGPU_E_HIP, "PCG made no progress (kernels not running)");
+    last_iter = ctx->h_sc->iter;
+  }
+  *iters = ctx->h_sc->iter;
+  *done_code = ctx->h_sc->done;
+  return FEMGPU_OK;
+}
+
+int done_status(int done) {
+  switch (done) {
+    case 1: return FEMGPU_OK;
+    case 2: return FEMGPU_E_NONFINITE;
+    case 3: return FEMGPU_E_BREAKDOWN;
+    case 4: return FEMGPU_E_MAXIT;
+    default: return FEMGPU_E_STATE;
+  }
+}
+
+int need_mesh(femgpu_ctx *ctx) {
+  if (!ctx) return FEMGPU_E_ARG;
+  if (!ctx->have_mesh) return fail(ctx, FEMGPU_E_STATE, "no mesh set");
+  HIPCHK(hipSetDevice(ctx->device));
+  return FEMGPU_OK;
+}
+
+// True relative residual ||f - K x|| / ||f|| in FP64 (x, f internal, device).
+int true_rel_res(femgpu_ctx *ctx, const double *f, const double *x, double *out) {
+  double *y, *r, *part, *part_node;
+  CgScalars *sc;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  int st;
+  if ((st = dbuf(ctx, "rr_y", nd, &y)) || (st = dbuf(ctx, "rr_r", nd, &r)) ||
+      (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part)) ||
+      (st = dbuf(ctx, "part_node", NODE_PART, &part_node)) ||
+      (st = dbuf(ctx, "sc_aux", 1, &sc)))
+    return st;
+  // ||f||^2 first: the residual of K x replaced by 0 (y and slots zero).
+  femgpu::launch_reset_scalars(sc, 1, 0.0, 0, ctx->stream);
+  femgpu::launch_fill<double>(y, int64_t(nd), 0.0, ctx->stream);
+  femgpu::launch_fill<double>(part, int64_t(ctx->plan.n_slots) * ctx->dim, 0.0, ctx->stream);
+  femgpu::launch_residual64(ctx->dm, f, y, part, r, sc, part_node, ctx->stream);
+  HIPCHK(hipGetLastError());
+  if ((st = poll(ctx, sc))) return st;
+  const double ff = ctx->h_sc->rr;
+  femgpu::launch_matvec_patch<double>(ctx->dm, x, y, part, ctx->stream);
+  femgpu::launch_residual64(ctx->dm, f, y, part, r, sc, part_node, ctx->stream);
+  HIPCHK(hipGetLastError());
+  if ((st = poll(ctx, sc))) return st;
+  *out = ff > 0 ? std::sqrt(ctx->h_sc->rr / ff) : 0.0;
+  return FEMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int femgpu_create(femgpu_ctx **out, int device_ordinal) {
+  if (!out) return FEMGPU_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FEMGPU_E_HIP;
+  if (device_ordinal < 0 || device_ordinal >= n) return FEMGPU_E_ARG;
+  std::unique_ptr<femgpu_ctx> c(new femgpu_ctx());
+  femgpu_ctx *ctx = c.get();
+  ctx->device = device_ordinal;
+  HIPCHK(hipSetDevice(device_ordinal));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  ctx->own_stream = true;
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_sc), sizeof(CgScalars), 0));
+  HIPCHK(hipEventCreate(&ctx->ev0));
+  HIPCHK(hipEventCreate(&ctx->ev1));
+  *out = c.release();
+  return FEMGPU_OK;
+}
+
+void femgpu_destroy(femgpu_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graphs(ctx);
+  for (auto &kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *femgpu_last_error(const femgpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int femgpu_set_stream(femgpu_ctx *ctx, void *hip_stream) {
+  if (!ctx) return FEMGPU_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->stream = nullptr;
+  ctx->own_stream = false;
+  if (hip_stream) {
+    ctx->stream = static_cast<hipStream_t>(hip_stream);
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  drop_graphs(ctx);
+  return FEMGPU_OK;
+}
+
+int femgpu_sync(femgpu_ctx *ctx) {
+  if (!ctx) return FEMGPU_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return FEMGPU_OK;
+}
+
+int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nodes,
+                    int64_t n_elems, const int32_t *elems) {
+  if (!ctx || !nodes || !elems) return FEMGPU_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  ctx->have_mesh = false;
+  const int npe = dim + 1;
+  int emax = ctx->emax;
+  if (emax <= 0) emax = dim == 3 ? 512 : 512;
+  std::string msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, ctx->plan);
+  if (!msg.empty()) return fail(ctx, FEMGPU_E_BAD_MESH, msg);
+  // degenerate elements
+  for (int64_t e = 0; e < n_elems; ++e) {
+    const int32_t *en = &elems[e * npe];
+    double det;
+    if (dim == 2) {
+      const double *a = &nodes[size_t(en[0]) * 2], *b = &nodes[size_t(en[1]) * 2],
+                   *c = &nodes[size_t(en[2]) * 2];
+      det = (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
+    } else {
+      const double *a = &nodes[size_t(en[0]) * 3], *b = &nodes[size_t(en[1]) * 3],
+                   *c = &nodes[size_t(en[2]) * 3], *d = &nodes[size_t(en[3]) * 3];
+      const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      const double e3[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+      det = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+            e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    }
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det))
+      return fail(ctx, FEMGPU_E_BAD_MESH, "degenerate element " + std::to_string(e));
+  }
+  ctx->dim = dim;
+  ctx->nn = n_nodes;
+  ctx->ne = n_elems;
+  ctx->nodes.assign(nodes, nodes + size_t(n_nodes) * dim);
+  ctx->elems.assign(elems, elems + size_t(n_elems) * npe);
+  ctx->escale.clear();
+  ctx->cnode.clear();
+  ctx->ccode.clear();
+  ctx->fnodes.clear();
+  ctx->fadj.clear();
+  ctx->fnormal.clear();
+  ctx->fp.clear();
+  ctx->lnode.clear();
+  ctx->lvec.clear();
+  ctx->extra_rhs.clear();
+  ctx->fix_user.assign(n_nodes, 0);
+  ctx->ctype.assign(n_nodes, 0);
+
+  const MeshPlan &P = ctx->plan;
+  DevMesh &dm = ctx->dm;
+  dm = DevMesh();
+  dm.dim = dim;
+  dm.np = int(P.patches.size());
+  dm.emax = P.emax_used;
+  dm.pn = P.pn_used;
+  dm.nn = P.nn;
+  dm.ne = P.ne;
+  dm.n_owned = P.n_owned;
+  dm.n_iface = P.n_iface;
+  dm.n_active = P.n_owned + P.n_iface;
+  int st;
+  femgpu::PatchInfo *pp;
+  uint32_t *lid;
+  int32_t *shg, *shs, *ss, *ei;
+  uint16_t *ne_, *inc;
+  if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "elem_lid", P.elem_lid, &lid)) ||
+      (st = upload(ctx, "sh_gid", P.sh_gid, &shg)) || (st = upload(ctx, "sh_slot", P.sh_slot, &shs)) ||
+      (st = upload(ctx, "node_end", P.node_end, &ne_)) || (st = upload(ctx, "inc", P.inc, &inc)) ||
+      (st = upload(ctx, "slot_start", P.slot_start, &ss)) ||
+      (st = upload(ctx, "elems_int", P.elems_int, &ei)))
+    return st;
+  dm.patches = pp;
+  dm.elem_lid = lid;
+  dm.sh_gid = shg;
+  dm.sh_slot = shs;
+  dm.node_end = ne_;
+  dm.inc = inc;
+  dm.slot_start = ss;
+  dm.elems_int = ei;
+  std::vector<double> c64(size_t(n_nodes) * dim);
+  std::vector<float> c32(c64.size());
+  for (int64_t i = 0; i < n_nodes; ++i)
+    for (int c = 0; c < dim; ++c) {
+      c64[size_t(i) * dim + c] = nodes[size_t(P.node_perm[i]) * dim + c];
+      c32[size_t(i) * dim + c] = float(c64[size_t(i) * dim + c]);
+    }
+  double *pc64;
+  float *pc32;
+  if ((st = upload(ctx, "coords64", c64, &pc64)) || (st = upload(ctx, "coords32", c32, &pc32)))
+    return st;
+  dm.coords64 = pc64;
+  dm.coords32 = pc32;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->have_mesh = true;
+  ctx->dirty_mat = ctx->dirty_fix = ctx->dirty_diag = true;
+  return ensure_fix(ctx);
+}
+
+int femgpu_set_material(femgpu_ctx *ctx, double E, double nu, const double *elem_scale) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!(E > 0) || !(nu > -1.0) || !(nu < (ctx->dim == 3 ? 0.5 : 1.0)))
+    return fail(ctx, FEMGPU_E_ARG, "invalid E / nu");
+  ctx->E = E;
+  ctx->nu = nu;
+  if (elem_scale) {
+    ctx->escale.assign(elem_scale, elem_scale + ctx->ne);
+    for (double v : ctx->escale)
+      if (!(v > 0) || !std::isfinite(v)) return fail(ctx, FEMGPU_E_ARG, "elem_scale must be > 0");
+  } else {
+    ctx->escale.clear();
+  }
+  ctx->dirty_mat = true;
+  return FEMGPU_OK;
+}
+
+int femgpu_set_constraints(femgpu_ctx *ctx, int64_t n, const int32_t *node, const int32_t *code) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (n < 0 || (n > 0 && (!node || !code))) return FEMGPU_E_ARG;
+  ctx->fix_user.assign(ctx->nn, 0);
+  ctx->ctype.assign(ctx->nn, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (node[i] < 0 || node[i] >= ctx->nn) return fail(ctx, FEMGPU_E_ARG, "constraint node out of range");
+    const int32_t t = code[i];
+    ctx->ctype[node[i]] = t;
+    uint8_t m = 0;
+    if (t & 1) m |= 1;
+    if (t & 2) m |= 2;
+    if (ctx->dim == 3 && (t & 3)) m |= 4;
+    ctx->fix_user[node[i]] |= m;
+  }
+  ctx->cnode.assign(node, node + n);
+  ctx->ccode.assign(code, code + n);
+  ctx->dirty_fix = true;
+  ctx->dirty_diag = true;
+  return FEMGPU_OK;
+}
+
+int femgpu_set_pressure_faces(femgpu_ctx *ctx, int64_t n, const int32_t *face_nodes,
+                              const int32_t *adj_elem, const double *normal, const double *pressure) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  const int d = ctx->dim, npe = d + 1;
+  if (n < 0 || (n > 0 && (!face_nodes || !adj_elem || !normal || !pressure))) return FEMGPU_E_ARG;
+  ctx->fnodes.assign(face_nodes, face_nodes + n * d);
+  ctx->fadj.assign(adj_elem, adj_elem + n);
+  ctx->fp.assign(pressure, pressure + n);
+  ctx->fnormal.resize(size_t(n) * d);
+  for (int64_t f = 0; f < n; ++f) {
+    if (adj_elem[f] < 0 || adj_elem[f] >= ctx->ne) return fail(ctx, FEMGPU_E_BAD_MESH, "face element out of range");
+    for (int i = 0; i < d; ++i) {
+      const int32_t v = face_nodes[f * d + i];
+      bool on = false;
+      for (int a = 0; a < npe; ++a) on |= ctx->elems[size_t(adj_elem[f]) * npe + a] == v;
+      if (!on) return fail(ctx, FEMGPU_E_BAD_MESH, "face node not on its element");
+    }
+    double l2 = 0;
+    for (int j = 0; j < d; ++j) l2 += normal[f * d + j] * normal[f * d + j];
+    if (!(l2 > 0)) return fail(ctx, FEMGPU_E_BAD_MESH, "zero face normal");
+    for (int j = 0; j < d; ++j) ctx->fnormal[f * d + j] = normal[f * d + j] / std::sqrt(l2);
+  }
+  return FEMGPU_OK;
+}
+
+int femgpu_set_nodal_loads(femgpu_ctx *ctx, int64_t n, const int32_t *node, const double *vec) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (n < 0 || (n > 0 && (!node || !vec))) return FEMGPU_E_ARG;
+  for (int64_t i = 0; i < n; ++i)
+    if (node[i] < 0 || node[i] >= ctx->nn) return fail(ctx, FEMGPU_E_ARG, "load node out of range");
+  ctx->lnode.assign(node, node + n);
+  ctx->lvec.assign(vec, vec + n * ctx->dim);
+  return FEMGPU_OK;
+}
+
+int femgpu_set_load_convention(femgpu_ctx *ctx, int convention) {
+  if (!ctx) return FEMGPU_E_ARG;
+  if (convention != FEMGPU_LOADS_REFERENCE && convention != FEMGPU_LOADS_PHYSICAL)
+    return fail(ctx, FEMGPU_E_ARG, "unknown load convention");
+  ctx->conv = convention;
+  return FEMGPU_OK;
+}
+
+int femgpu_set_extra_rhs(femgpu_ctx *ctx, const double *f) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (f)
+    ctx->extra_rhs.assign(f, f + size_t(ctx->nn) * ctx->dim);
+  else
+    ctx->extra_rhs.clear();
+  return FEMGPU_OK;
+}
+
+int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!x || !y) return FEMGPU_E_ARG;
+  if ((st = ensure_material(ctx)) || (st = ensure_fix(ctx))) return st;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  std::vector<double> xi = to_internal(ctx, x), yi(nd, 0.0);
+  double *dx, *dy, *part;
+  if ((st = upload(ctx, "mv_x", xi, &dx)) || (st = dbuf(ctx, "mv_y", nd, &dy)) ||
+      (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part)))
+    return st;
+  femgpu::launch_fill<double>(dy, int64_t(nd), 0.0, ctx->stream);
+  if (precision == FEMGPU_FP64) {
+    femgpu::launch_matvec<double>(ctx->dm, dx, dy, part, ctx->stream);
+  } else if (precision == FEMGPU_FP32) {
+    float *fx, *fy, *fpart;
+    if ((st = dbuf(ctx, "mv_x32", nd, &fx)) || (st = dbuf(ctx, "mv_y32", nd, &fy)) ||
+        (st = dbuf(ctx, "partial32", size_t(ctx->plan.n_slots) * ctx->dim + 1, &fpart)))
+      return st;
+    femgpu::launch_convert<double, float>(dx, fx, int64_t(nd), ctx->stream);
+    femgpu::launch_fill<float>(fy, int64_t(nd), 0.f, ctx->stream);
+    femgpu::launch_matvec<float>(ctx->dm, fx, fy, fpart, ctx->stream);
+    femgpu::launch_convert<float, double>(fy, dy, int64_t(nd), ctx->stream);
+  } else {
+    return fail(ctx, FEMGPU_E_ARG, "precision must be FP64 or FP32");
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(yi.data(), dy, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  from_internal(ctx, yi, y);
+  return FEMGPU_OK;
+}
+
+int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if ((st = ensure_material(ctx)) || (st = ensure_fix(ctx))) return st;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  if (precision == FEMGPU_FP64) {
+    double *dx, *dy, *part;
+    if ((st = dbuf(ctx, "mv_x", nd, &dx)) || (st = dbuf(ctx, "mv_y", nd, &dy)) ||
+        (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part)))
+      return st;
+    for (int k = 0; k < reps; ++k) femgpu::launch_matvec<double>(ctx->dm, dx, dy, part, ctx->stream);
+  } else {
+    float *fx, *fy, *fpart;
+    if ((st = dbuf(ctx, "mv_x32", nd, &fx)) || (st = dbuf(ctx, "mv_y32", nd, &fy)) ||
+        (st = dbuf(ctx, "partial32", size_t(ctx->plan.n_slots) * ctx->dim + 1, &fpart)))
+      return st;
+    for (int k = 0; k < reps; ++k) femgpu::launch_matvec<float>(ctx->dm, fx, fy, fpart, ctx->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return FEMGPU_OK;
+}
+
+int femgpu_diag(femgpu_ctx *ctx, double *d) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!d) return FEMGPU_E_ARG;
+  if ((st = ensure_diag(ctx))) return st;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  std::vector<double> v(nd);
+  double *dd;
+  if ((st = dbuf(ctx, "diag64", nd, &dd))) return st;
+  HIPCHK(hipMemcpyAsync(v.data(), dd, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  from_internal(ctx, v, d);
+  return FEMGPU_OK;
+}
+
+int femgpu_rhs(femgpu_ctx *ctx, double *f) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!f) return FEMGPU_E_ARG;
+  std::vector<double> v;
+  host_rhs(ctx, false, v);
+  std::copy(v.begin(), v.end(), f);
+  return FEMGPU_OK;
+}
+
+int femgpu_default_solve_opts(femgpu_solve_opts *o) {
+  if (!o) return FEMGPU_E_ARG;
+  o->precision = FEMGPU_FP64;
+  o->tol_kind = FEMGPU_TOL_REF_GAMMA;
+  o->tol = 1e-8;
+  o->max_iter = 0;
+  o->inner_tol = 1e-4;
+  o->check_every = 32;
+  return FEMGPU_OK;
+}
+
+int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
+                        femgpu_stats *stats) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  femgpu_solve_opts o;
+  femgpu_default_solve_opts(&o);
+  if (opts) o = *opts;
+  if (!(o.tol >= 0) || (o.tol_kind != FEMGPU_TOL_REF_GAMMA && o.tol_kind != FEMGPU_TOL_REL_RES))
+    return fail(ctx, FEMGPU_E_ARG, "bad tolerance");
+  if (o.precision == FEMGPU_MIXED && o.tol_kind != FEMGPU_TOL_REL_RES)
+    return fail(ctx, FEMGPU_E_ARG, "MIXED precision needs FEMGPU_TOL_REL_RES");
+  const int max_iter = o.max_iter > 0 ? o.max_iter : 1000000;
+  const int block = o.check_every > 0 ? o.check_every : 32;
+  if ((st = ensure_diag(ctx))) return st;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  std::vector<double> fu, fi;
+  host_rhs(ctx, false, fu);
+  fi = to_internal(ctx, fu.data());
+  double *f64, *x64;
+  CgScalars *sc, *sc2;
+  if ((st = upload(ctx, "f64", fi, &f64)) || (st = dbuf(ctx, "sol64", nd, &x64)) ||
+      (st = dbuf(ctx, "sc_main", 1, &sc)) || (st = dbuf(ctx, "sc_outer", 1, &sc2)))
+    return st;
+  femgpu_stats S{};
+  int iters = 0, done = 0;
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  if (o.precision == FEMGPU_FP64) {
+    PcgWs<double> w;
+    if ((st = pcg_ws(ctx, "c64", w))) return st;
+    w.invD = static_cast<double *>(ctx->bufs["invD64"].p);
+    femgpu::launch_copy<double>(f64, w.r, int64_t(nd), ctx->stream);
+    if ((st = run_pcg<double>(ctx, w, OpShift<double>(), o.tol_kind, o.tol, max_iter, block, 0,
+                              sc, &iters, &done)))
+      return st;
+    femgpu::launch_copy<double>(w.x, x64, int64_t(nd), ctx->stream);
+    S.iterations = iters;
+    S.gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->scale / ctx->h_sc->gamma0 : 0.0;
+    S.status = done_status(done);
+  } else if (o.precision == FEMGPU_FP32) {
+    PcgWs<float> w;
+    if ((st = pcg_ws(ctx, "c32", w))) return st;
+    w.invD = static_cast<float *>(ctx->bufs["invD32"].p);
+    femgpu::launch_convert<double, float>(f64, w.r, int64_t(nd), ctx->stream);
+    if ((st = run_pcg<float>(ctx, w, OpShift<float>(), o.tol_kind, o.tol, max_iter, block, 1, sc,
+                             &iters, &done)))
+      return st;
+    femgpu::launch_convert<float, double>(w.x, x64, int64_t(nd), ctx->stream);
+    S.iterations = iters;
+    S.gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->scale / ctx->h_sc->gamma0 : 0.0;
+    S.status = done_status(done);
+  } else if (o.precision == FEMGPU_MIXED) {
+    // FP32 PCG inside FP64 iterative refinement: r = f - K x in FP64, solve
+    // K d = r / ||r|| in FP32 to inner_tol, x += ||r|| d.
+    PcgWs<float> w;
+    double *r64, *y64, *part64, *part_node;
+    if ((st = pcg_ws(ctx, "c32", w)) || (st = dbuf(ctx, "ir_r", nd, &r64)) ||
+        (st = dbuf(ctx, "ir_y", nd, &y64)) ||
+        (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part64)) ||
+        (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
+      return st;
+    w.invD = static_cast<float *>(ctx->bufs["invD32"].p);
+    femgpu::launch_fill<double>(x64, int64_t(nd), 0.0, ctx->stream);
+    femgpu::launch_fill<double>(y64, int64_t(nd), 0.0, ctx->stream);
+    femgpu::launch_reset_scalars(sc2, 1, 0.0, 0, ctx->stream);
+    // ||f||: residual of x = 0 (y = 0 everywhere, no partial slots read for
+    // owned nodes; interface slots cleared by a product of x = 0).
+    femgpu::launch_matvec_patch<double>(ctx->dm, x64, y64, part64, ctx->stream);
+    femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
+    if ((st = poll(ctx, sc2))) return st;
+    const double fnorm = std::sqrt(ctx->h_sc->rr);
+    double rel = fnorm > 0 ? 1.0 : 0.0;
+    int outer = 0;
+    int status = FEMGPU_OK;
+    const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+    while (rel > o.tol && outer < 50) {
+      femgpu::launch_to_f32_scaled(ctx->dm, r64, w.r, w.x, sc2, ctx->stream);
+      int it = 0;
+      if ((st = run_pcg<float>(ctx, w, OpShift<float>(), FEMGPU_TOL_REL_RES, inner,
+                               std::max(1, max_iter - iters), block, 1, sc, &it, &done)))
+        return st;
+      iters += it;
+      if (done == 2 || done == 3) {
+        status = done_status(done);
+        break;
+      }
+      femgpu::launch_axpy_back(ctx->dm, x64, w.x, sc2, ctx->stream);
+      femgpu::launch_matvec_patch<double>(ctx->dm, x64, y64, part64, ctx->stream);
+      femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
+      HIPCHK(hipGetLastError());
+      if ((st = poll(ctx, sc2))) return st;
+      rel = fnorm > 0 ? std::sqrt(ctx->h_sc->rr) / fnorm : 0.0;
+      ++outer;
+      if (iters >= max_iter && rel > o.tol) {
+        status = FEMGPU_E_MAXIT;
+        break;
+      }
+    }
+    if (status == FEMGPU_OK && rel > o.tol) status = FEMGPU_E_MAXIT;
+    S.iterations = iters;
+    S.outer_iterations = outer;
+    S.status = status;
+  } else {
+    return fail(ctx, FEMGPU_E_ARG, "unknown precision");
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  S.solve_ms = ms;
+  double rr;
+  if ((st = true_rel_res(ctx, f64, x64, &rr))) return st;
+  S.rel_res = rr;
+  if (u_out) {
+    std::vector<double> xi(nd);
+    HIPCHK(hipMemcpyAsync(xi.data(), x64, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    from_internal(ctx, xi, u_out);
+  }
+  if (stats) *stats = S;
+  if (S.status != FEMGPU_OK) {
+    ctx->err = "solve stopped with status " + std::to_string(S.status);
+    return S.status;
+  }
+  return FEMGPU_OK;
+}
+
+int femgpu_stress(femgpu_ctx *ctx, const double *u, double *sigma_out) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!u || !sigma_out) return FEMGPU_E_ARG;
+  if ((st = ensure_material(ctx))) return st;
+  const int nv = 3 * (ctx->dim - 1);
+  std::vector<double> ui = to_internal(ctx, u), s(size_t(ctx->ne) * nv);
+  double *du, *ds;
+  if ((st = upload(ctx, "st_u", ui, &du)) || (st = dbuf(ctx, "st_s", s.size(), &ds))) return st;
+  femgpu::launch_stress<double>(ctx->dm, du, ds, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(s.data(), ds, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < ctx->ne; ++i)
+    for (int k = 0; k < nv; ++k) sigma_out[size_t(ctx->plan.elem_perm[i]) * nv + k] = s[size_t(i) * nv + k];
+  return FEMGPU_OK;
+}
+
+int femgpu_element_energy(femgpu_ctx *ctx, const double *u, double *e_out) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!u || !e_out) return FEMGPU_E_ARG;
+  if ((st = ensure_material(ctx)) || (st = ensure_fix(ctx))) return st;
+  std::vector<double> ui = to_internal(ctx, u), e(ctx->ne);
+  double *du, *de;
+  if ((st = upload(ctx, "st_u", ui, &du)) || (st = dbuf(ctx, "en_e", e.size(), &de))) return st;
+  femgpu::launch_energy<double>(ctx->dm, du, de, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(e.data(), de, e.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < ctx->ne; ++i) e_out[ctx->plan.elem_perm[i]] = e[i];
+  return FEMGPU_OK;
+}
+
+int femgpu_default_dyn_opts(femgpu_dyn_opts *o) {
+  if (!o) return FEMGPU_E_ARG;
+  o->rho = 2400;
+  o->beta1 = 0.5;
+  o->beta2 = 0.0;
+  o->dt = 1e-3;
+  o->endtime = 0.1;
+  o->wavelet_type = FEMGPU_WAVELET_NONE;
+  o->wavelet_freq = 15;
+  o->wavelet_timeshift = 0;
+  o->wavelet_ampl = 1;
+  o->precision = FEMGPU_FP64;
+  o->mass_convention = FEMGPU_MASS_REFERENCE;
+  o->implicit_mode = FEMGPU_IMPLICIT_REFERENCE;
+  o->pcg_eps = 1e-4;
+  o->steps_per_graph = 64;
+  return FEMGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-DOF assembled mass (user order).  lumped: 2D rho A/3, 3D rho V/12
+// (reference) or rho V/4 (physical); consistent diagonal: 2D rho A/6, 3D rho V/24.
+std::vector<double> host_mass(const femgpu_ctx *ctx, double rho, bool lumped, int conv) {
+  const int d = ctx->dim, npe = d + 1;
+  std::vector<double> m(size_t(ctx->nn) * d, 0.0);
+  for (int64_t e = 0; e < ctx->ne; ++e) {
+    const int32_t *en = &ctx->elems[size_t(e) * npe];
+    double meas;
+    if (d == 2) {
+      const double *a = &ctx->nodes[size_t(en[0]) * 2], *b = &ctx->nodes[size_t(en[1]) * 2],
+                   *c = &ctx->nodes[size_t(en[2]) * 2];
+      meas = 0.5 * std::fabs((b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]));
+    } else {
+      const double *a = &ctx->nodes[size_t(en[0]) * 3], *b = &ctx->nodes[size_t(en[1]) * 3],
+                   *c = &ctx->nodes[size_t(en[2]) * 3], *q = &ctx->nodes[size_t(en[3]) * 3];
+      const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      const double e3[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+      meas = std::fabs(e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) -
+                       e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                       e1[2] * (e2[0] * e3[1] - e2[1] * e3[0])) / 6.0;
+    }
+    const double mass = rho * meas;
+    double per;
+    if (d == 2)
+      per = lumped ? mass / 3.0 : mass / 6.0;
+    else if (lumped)
+      per = conv == FEMGPU_MASS_PHYSICAL ? mass / 4.0 : mass / 12.0;
+    else
+      per = mass / 24.0;
+    for (int a = 0; a < npe; ++a)
+      for (int c = 0; c < d; ++c) m[size_t(en[a]) * d + c] += per;
+  }
+  return m;
+}
+
+template <typename T>
+int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_dyn_stats *stats) {
+  const int d = ctx->dim;
+  const size_t nd = size_t(ctx->nn) * d;
+  const bool expl = o.beta2 == 0.0;
+  int st;
+  if ((st = ensure_diag(ctx))) return st;
+  // Time grid and RHS coefficients follow the reference's float arithmetic:
+  // endnt = int(endtime/dt), t = nt*dt, pressures overwritten by the wavelet
+  // value once it is non-zero (cpu_fem_utils.cpp:564-569).
+  const float dtf = float(o.dt), endf = float(o.endtime);
+  const int endnt = endf < 0.f ? 1 : int(endf / dtf);
+  if (endnt < 1) return fail(ctx, FEMGPU_E_ARG, "endtime/dt gives no step");
+  std::vector<double> cfile(endnt + 1, 1.0), cunit(endnt + 1, 0.0);
+  {
+    const bool loads = ctx->conv == FEMGPU_LOADS_REFERENCE && !ctx->lnode.empty();
+    const float f = float(o.wavelet_freq), ts = float(o.wavelet_timeshift), am = float(o.wavelet_ampl);
+    float last = wavelet_value(0.f, o.wavelet_type, f, ts, am);
+    for (int nt = 1; nt <= endnt; ++nt) {
+      const float t = float(nt) * dtf;
+      const float v = wavelet_value(t, o.wavelet_type, f, ts, am);
+      if (v != 0.f) last = v;
+      if (!loads && last != 0.f) {
+        cfile[nt] = 0.0;
+        cunit[nt] = double(last);
+      }
+    }
+  }
+  std::vector<double> ffile, funit;
+  host_rhs(ctx, false, ffile);
+  host_rhs(ctx, true, funit);
+  std::vector<double> ff_i = to_internal(ctx, ffile.data()), fu_i = to_internal(ctx, funit.data());
+  std::vector<T> ff_t(ff_i.begin(), ff_i.end()), fu_t(fu_i.begin(), fu_i.end());
+  const int mconv = o.mass_convention;
+  std::vector<double> mass_l = host_mass(ctx, o.rho, true, mconv);
+  std::vector<double> ml_i = to_internal(ctx, mass_l.data());
+  std::vector<T> invM(nd, T(0)), mlump(nd, T(0));
+  for (size_t i = 0; i < nd; ++i) {
+    mlump[i] = T(ml_i[i]);
+    invM[i] = ml_i[i] > 0 ? T(1.0 / ml_i[i]) : T(0);
+  }
+  const std::string tg = sizeof(T) == 8 ? "d64" : "d32";
+  T *dd, *dv, *da, *dy, *dpart, *dff, *dfu, *dinvM, *dml;
+  double *dcf, *dcu;
+  int *dstep;
+  if ((st = dbuf(ctx, tg + "d", nd, &dd)) || (st = dbuf(ctx, tg + "v", nd, &dv)) ||
+      (st = dbuf(ctx, tg + "a", nd, &da)) || (st = dbuf(ctx, tg + "y", nd, &dy)) ||
+      (st = dbuf(ctx, tg + "part", size_t(ctx->plan.n_slots) * d + 1, &dpart)) ||
+      (st = upload(ctx, tg + "ff", ff_t, &dff)) || (st = upload(ctx, tg + "fu", fu_t, &dfu)) ||
+      (st = upload(ctx, tg + "invM", invM, &dinvM)) || (st = upload(ctx, tg + "ml", mlump, &dml)) ||
+      (st = upload(ctx, "dyn_cf", cfile, &dcf)) || (st = upload(ctx, "dyn_cu", cunit, &dcu)) ||
+      (st = dbuf(ctx, "dyn_step", 1, &dstep)))
+    return st;
+  femgpu::launch_fill<T>(dd, int64_t(nd), T(0), ctx->stream);
+  femgpu::launch_fill<T>(dv, int64_t(nd), T(0), ctx->stream);
+  femgpu::launch_fill<T>(da, int64_t(nd), T(0), ctx->stream);
+  femgpu::launch_fill<T>(dy, int64_t(nd), T(0), ctx->stream);
+  // GN22 coefficients in float as the reference forms them (femfunc.cpp:258-310).
+  const float b1 = float(o.beta1), b2 = float(o.beta2);
+  femgpu::DynCoeffs C;
+  C.dt = double(dtf);
+  C.c1 = double(0.5f * (1.f - b2) * dtf * dtf);
+  C.c2 = double((1.f - b1) * dtf);
+  C.c3 = double(b1 * dtf);
+  C.c4 = double(0.5f * b2 * dtf * dtf);
+  femgpu_dyn_stats S{};
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  if (expl) {
+    const int G = o.steps_per_graph > 0 ? o.steps_per_graph : 64;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < G; ++k) {
+      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
+      femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+      femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM, da,
+                                    dv, nullptr, C, ctx->stream);
+    }
+    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+    HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    int done = 0;
+    while (done + G <= endnt) {
+      femgpu::launch_set_int(dstep, done + 1, ctx->stream);
+      HIPCHK(hipGraphLaunch(exec, ctx->stream));
+      done += G;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGraphExecDestroy(exec);
+    femgpu::launch_set_int(dstep, done + 1, ctx->stream);
+    for (int k = 0; done < endnt; ++k, ++done) {
+      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
+      femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+      femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM, da,
+                                    dv, nullptr, C, ctx->stream);
+    }
+    HIPCHK(hipGetLastError());
+    S.steps = endnt;
+  } else {
+    // implicit: nested PCG per step on A a = F - K d.
+    PcgWs<T> w;
+    if ((st = pcg_ws(ctx, sizeof(T) == 8 ? "c64" : "c32", w))) return st;
+    T *invA;
+    if ((st = dbuf(ctx, tg + "invA", nd, &invA))) return st;
+    OpShift<T> sh;
+    const T ceff = T(C.c4);  // 0.5 beta2 dt^2
+    const T *diagK_t;
+    if (sizeof(T) == 8) {
+      diagK_t = reinterpret_cast<const T *>(ctx->bufs["diag64"].p);
+    } else {
+      float *d32;
+      if ((st = dbuf(ctx, "diag32", nd, &d32))) return st;
+      femgpu::launch_convert<double, float>(static_cast<double *>(ctx->bufs["diag64"].p), d32,
+                                            int64_t(nd), ctx->stream);
+      diagK_t = reinterpret_cast<const T *>(d32);
+    }
+    if (o.implicit_mode == FEMGPU_IMPLICIT_PHYSICAL) {
+      sh.cK = ceff;
+      sh.cM = T(1);
+      sh.m = dml;
+      femgpu::launch_inv_diag<T>(ctx->dm, diagK_t, ceff, T(1), dml, invA, ctx->stream);
+    } else {
+      // reference: operator K, preconditioner cK diag(K) + diag(M_consistent)
+      std::vector<double> mc = host_mass(ctx, o.rho, false, mconv);
+      std::vector<double> mc_i = to_internal(ctx, mc.data());
+      std::vector<T> mc_t(mc_i.begin(), mc_i.end());
+      T *dmc;
+      if ((st = upload(ctx, tg + "mc", mc_t, &dmc))) return st;
+      femgpu::launch_inv_diag<T>(ctx->dm, diagK_t, ceff, T(1), dmc, invA, ctx->stream);
+    }
+    w.invD = invA;
+    CgScalars *sc;
+    if ((st = dbuf(ctx, "sc_main", 1, &sc))) return st;
+    int total_it = 0;
+    for (int nt = 1; nt <= endnt; ++nt) {
+      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, true, ctx->stream);
+      femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+      femgpu::launch_set_int(dstep, nt, ctx->stream);
+      femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, 0, nullptr, da,
+                                    dv, w.r, C, ctx->stream);
+      int it = 0, done = 0;
+      if ((st = run_pcg<T>(ctx, w, sh, FEMGPU_TOL_REF_GAMMA, o.pcg_eps, 1000000, 8,
+                           sizeof(T) == 8 ? 10 : 11, sc, &it, &done)))
+        return st;
+      total_it += it;
+      if (done != 1) {
+        S.status = done_status(done);
+        ctx->err = "implicit step " + std::to_string(nt) + ": PCG stopped with " + std::to_string(done);
+        if (stats) *stats = S;
+        return S.status;
+      }
+      femgpu::launch_copy<T>(w.x, da, int64_t(nd), ctx->stream);
+      femgpu::launch_dyn_implicit_post<T>(ctx->dm, da, dv, dd, C, ctx->stream);
+    }
+    HIPCHK(hipGetLastError());
+    S.steps = endnt;
+    S.pcg_iterations = total_it;
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  S.run_ms = ms;
+  if (u_out) {
+    std::vector<T> xt(nd);
+    HIPCHK(hipMemcpyAsync(xt.data(), dd, nd * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> xi(xt.begin(), xt.end());
+    from_internal(ctx, xi, u_out);
+  }
+  S.status = FEMGPU_OK;
+  if (stats) *stats = S;
+  return FEMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int femgpu_dyn_run(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, double *u_out,
+                   femgpu_dyn_stats *stats) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  femgpu_dyn_opts o;
+  femgpu_default_dyn_opts(&o);
+  if (opts) o = *opts;
+  if (!(o.dt > 0) || !(o.rho > 0)) return fail(ctx, FEMGPU_E_ARG, "dt and rho must be > 0");
+  if (o.precision == FEMGPU_FP64) return dyn_run_t<double>(ctx, o, u_out, stats);
+  if (o.precision == FEMGPU_FP32) return dyn_run_t<float>(ctx, o, u_out, stats);
+  return fail(ctx, FEMGPU_E_ARG, "dynamics precision must be FP64 or FP32");
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// kernels.h -- launch interface of the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "femgpu_internal.h"
+
+namespace femgpu {
+
+// Device view of a planned mesh (all pointers are device pointers).
+struct DevMesh {
+  int dim = 0;
+  int np = 0;              // patches
+  int emax = 0;            // LDS stride (elements per patch bound)
+  int pn = 0;              // LDS node capacity
+  int64_t nn = 0, ne = 0, n_owned = 0, n_iface = 0, n_active = 0;
+  const PatchInfo *patches = nullptr;
+  const uint32_t *elem_lid = nullptr;
+  const int32_t *sh_gid = nullptr;
+  const int32_t *sh_slot = nullptr;
+  const uint16_t *node_end = nullptr;
+  const uint16_t *inc = nullptr;
+  const int32_t *slot_start = nullptr;
+  const uint8_t *fixmask = nullptr;   // per internal node: bit c = component c fixed
+  const int32_t *elems_int = nullptr; // ne x npe internal node ids
+  const double *coords64 = nullptr;
+  const float *coords32 = nullptr;
+  const double *escale64 = nullptr;   // per internal element or null
+  const float *escale32 = nullptr;
+  double lam = 0, mu = 0;             // Lame constants (2D: plane stress)
+};
+
+// Scalars of one PCG run, device resident.
+struct CgScalars {
+  double gamma;        // current r'z
+  double gamma0;
+  double alpha;
+  double beta;
+  double pq;
+  double rr;           // r'r after the last update
+  double rr0;
+  double thresh;       // stop threshold (kind dependent)
+  double eps;          // tol as given
+  int kind;            // 0: gamma ratio, 1: relative residual
+  int done;            // 0 running, 1 converged, 2 non-finite, 3 breakdown, 4 max-iter
+  int iter;
+  int max_iter;
+  unsigned cnt_a;      // last-block tickets
+  unsigned cnt_b;
+  double scale;        // scratch scalar (norms for refinement)
+};
+
+// Optional mass shift of the operator: y = cK*K x + cM*diag(m) x.
+template <typename T>
+struct OpShift {
+  T cK = T(1);
+  T cM = T(0);
+  const T *m = nullptr;   // per DOF, internal order
+};
+
+int lds_bytes(int dim, int elem_bytes, int pn, int emax);
+
+// y = K x (standalone: patch kernel + interface gather).
+template <typename T>
+void launch_matvec(const DevMesh &dm, const T *x, T *y, T *partial, hipStream_t s);
+// Patch kernel only (interface results left in `partial`).
+template <typename T>
+void launch_matvec_patch(const DevMesh &dm, const T *x, T *y, T *partial, hipStream_t s);
+// Assembled diagonal of K.
+template <typename T>
+void launch_diag(const DevMesh &dm, T *d, T *partial, hipStream_t s);
+
+// invD = 1 / (cK*diag + cM*m) (0 where the sum is 0).
+template <typename T>
+void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T *invD,
+                     hipStream_t s);
+
+// PCG: init (z = invD r, gamma0) and one fused iteration.
+template <typename T>
+void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScalars *sc,
+                     double *part, hipStream_t s);
+template <typename T>
+void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial,
+                     const T *invD, OpShift<T> sh, CgScalars *sc, double *part_a,
+                     double *part_b, hipStream_t s);
+
+// r = f - (interface-finished) K x, rr -> sc->rr (FP64 outer loop of MIXED).
+void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
+                       double *r, CgScalars *sc, double *part, hipStream_t s);
+// r32 = r64 / sqrt(sc->rr); x32 = 0.
+void launch_to_f32_scaled(const DevMesh &dm, const double *r64, float *r32, float *x32,
+                          const CgScalars *sc, hipStream_t s);
+// x64 += sqrt(sc->rr) * x32.
+void launch_axpy_back(const DevMesh &dm, double *x64, const float *x32, const CgScalars *sc,
+                      hipStream_t s);
+
+template <typename T>
+void launch_fill(T *p, int64_t n, T v, hipStream_t s);
+template <typename T>
+void launch_copy(const T *a, T *b, int64_t n, hipStream_t s);
+template <typename TA, typename TB>
+void launch_convert(const TA *a, TB *b, int64_t n, hipStream_t s);
+
+// Per-element outputs in internal element order.
+template <typename T>
+void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s);
+template <typename T>
+void launch_energy(const DevMesh &dm, const T *u, T *en, hipStream_t s);
+
+// Dynamics.
+struct DynCoeffs {
+  double dt, c1, c2, c3, c4;  // predictor / corrector coefficients
+};
+template <typename T>
+void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
+                        hipStream_t s);
+// r = F(step) - K d (interface finished from partials); explicit: a = r*invM, v += c3 a.
+template <typename T>
+void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T *f_file,
+                        const T *f_unit, const double *coef_file, const double *coef_unit,
+                        const int *step_base, int step_off, const T *invM, T *a, T *v,
+                        T *r_out, DynCoeffs c, hipStream_t s);
+template <typename T>
+void launch_dyn_implicit_post(const DevMesh &dm, const T *a, T *v, T *d, DynCoeffs c,
+                              hipStream_t s);
+void launch_set_int(int *p, int v, hipStream_t s);
+void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s);
+
+}  // namespace femgpu

@@ -1,0 +1,1168 @@
+// kernels.hip -- hand-written HIP kernels (gfx950) of the EbE elasticity path.
+//
+// What replaces what in the reference (moarseniy/CudaFEMproject):
+//  * k_ebe_patch replaces, per PCG iteration, Matrix::bmm with the stored
+//    per-element K_e (cuBLAS SgemmStridedBatched, cuda_matrix.cpp:252-275) plus
+//    the sort-based assembly reductionWithMask{,AndTransform}
+//    (cuda_fem_utils.cpp:196-216, thrust sort_by_key/reduce_by_key,
+//    matrix_kernels.cu:37-51).  K_e = V B'DB is never stored: each element's
+//    force K_e u_e is formed on the fly from the node coordinates, one element
+//    per thread, with the patch's node data staged in LDS, and assembled in
+//    LDS through fixed-order incidence lists.  Interface nodes are finished by
+//    a fixed-order sum over per-patch partial slots.
+//  * k_pcg_update fuses the PCG vector updates and both dot products of an
+//    iteration (femfunc.cpp:59-111: addWeighted x3, divideElementwise,
+//    dotProduct x2) into one pass; the dot products reduce with wave64
+//    shuffles and a fixed-order last-block sum, and the scalars never leave
+//    the device.
+//  * The element-force math is the reference's K_e = B'DB * area * coeff
+//    (cpu_fem_utils.cpp:338-503) rewritten with the barycentric gradients
+//    g_a = c_a / det J (c_a: cofactor vectors), so that
+//    f_a = (1 / (2|det J|) or 1 / (6|det J|)) * S(H) c_a, H = sum_a du_a (x) c_a,
+//    S(H) = lam tr(H) I + mu (H + H').
+//  * Constrained DOFs follow applyConstraints (cpu_fem_utils.cpp:461-476):
+//    off-diagonal couplings of a fixed DOF are dropped and its diagonal kept.
+// All reductions have a fixed order, so results are bitwise reproducible
+// run to run for a given mesh plan.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace femgpu {
+
+namespace {
+
+constexpr int NT = 256;          // threads per block for every kernel
+constexpr int NODE_GRID_MAX = 2048;
+
+struct LdsLayout {
+  int off_c, off_x, off_res, off_end, off_inc, off_fix, off_red, off_flag, total;
+};
+
+__host__ __device__ inline int al16(int v) { return (v + 15) & ~15; }
+
+__host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int emax) {
+  const int npe = dim + 1, nde = dim * npe;
+  LdsLayout L;
+  L.off_c = 0;
+  L.off_x = al16(pn * dim * tb);
+  L.off_res = L.off_x + al16(pn * dim * tb);
+  L.off_end = L.off_res + al16(emax * nde * tb);
+  L.off_inc = L.off_end + al16(pn * 2);
+  L.off_fix = L.off_inc + al16(emax * 4 * 2);
+  L.off_red = L.off_fix + al16(pn);
+  L.off_flag = L.off_red + al16((NT / 64) * 8);
+  L.total = L.off_flag + 16;
+  return L;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum, same value returned in every thread; fixed order.
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) r += sh[i];
+  return r;
+}
+
+__device__ __forceinline__ void store_agent(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double *p) {
+  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread 0 of each block publishes its partial first; returns true in all
+// threads of the block that arrives last (it then owns the final reduction).
+__device__ __forceinline__ bool arrive_last(unsigned *counter, unsigned total, int *s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    *s_flag = (t == total - 1);
+  }
+  __syncthreads();
+  const bool last = *s_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// ---------------------------------------------------------------------------
+// Element kernels (registers only).  X, U: node-major (npe x dim).
+// fix: bit (dim*a + c) set when that local DOF is constrained.
+// Returns u'K u of the (constraint-modified) element matrix.
+
+template <typename T>
+__device__ __forceinline__ T tet_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
+                                       T *F) {
+  const T e1x = X[3] - X[0], e1y = X[4] - X[1], e1z = X[5] - X[2];
+  const T e2x = X[6] - X[0], e2y = X[7] - X[1], e2z = X[8] - X[2];
+  const T e3x = X[9] - X[0], e3y = X[10] - X[1], e3z = X[11] - X[2];
+  T c[4][3];
+  c[1][0] = e2y * e3z - e2z * e3y;
+  c[1][1] = e2z * e3x - e2x * e3z;
+  c[1][2] = e2x * e3y - e2y * e3x;
+  c[2][0] = e3y * e1z - e3z * e1y;
+  c[2][1] = e3z * e1x - e3x * e1z;
+  c[2][2] = e3x * e1y - e3y * e1x;
+  c[3][0] = e1y * e2z - e1z * e2y;
+  c[3][1] = e1z * e2x - e1x * e2z;
+  c[3][2] = e1x * e2y - e1y * e2x;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) c[0][j] = -(c[1][j] + c[2][j] + c[3][j]);
+  const T det = e1x * c[1][0] + e1y * c[1][1] + e1z * c[1][2];
+  const T s = es / (T(6) * fabs(det));
+  T um[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) um[i] = ((fix >> i) & 1u) ? T(0) : U[i];
+  T H[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) H[i][j] = T(0);
+#pragma unroll
+  for (int a = 1; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const T du = um[3 * a + i] - um[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) H[i][j] += du * c[a][j];
+    }
+  const T tr = H[0][0] + H[1][1] + H[2][2];
+  const T S00 = lam * tr + T(2) * mu * H[0][0];
+  const T S11 = lam * tr + T(2) * mu * H[1][1];
+  const T S22 = lam * tr + T(2) * mu * H[2][2];
+  const T S01 = mu * (H[0][1] + H[1][0]);
+  const T S02 = mu * (H[0][2] + H[2][0]);
+  const T S12 = mu * (H[1][2] + H[2][1]);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    F[3 * a + 0] = s * (S00 * c[a][0] + S01 * c[a][1] + S02 * c[a][2]);
+    F[3 * a + 1] = s * (S01 * c[a][0] + S11 * c[a][1] + S12 * c[a][2]);
+    F[3 * a + 2] = s * (S02 * c[a][0] + S12 * c[a][1] + S22 * c[a][2]);
+  }
+  if (fix) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+      if ((fix >> i) & 1u) {
+        const int a = i / 3, cc = i % 3;
+        const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1] + c[a][2] * c[a][2];
+        F[i] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2) * U[i];
+      }
+  }
+  T en = T(0);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) en += U[i] * F[i];
+  return en;
+}
+
+template <typename T>
+__device__ __forceinline__ T tri_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
+                                       T *F) {
+  const T e1x = X[2] - X[0], e1y = X[3] - X[1];
+  const T e2x = X[4] - X[0], e2y = X[5] - X[1];
+  T c[3][2];
+  c[1][0] = e2y;
+  c[1][1] = -e2x;
+  c[2][0] = -e1y;
+  c[2][1] = e1x;
+  c[0][0] = -(c[1][0] + c[2][0]);
+  c[0][1] = -(c[1][1] + c[2][1]);
+  const T det = e1x * e2y - e1y * e2x;
+  const T s = es / (T(2) * fabs(det));
+  T um[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) um[i] = ((fix >> i) & 1u) ? T(0) : U[i];
+  T H00 = T(0), H01 = T(0), H10 = T(0), H11 = T(0);
+#pragma unroll
+  for (int a = 1; a < 3; ++a) {
+    const T dx = um[2 * a] - um[0], dy = um[2 * a + 1] - um[1];
+    H00 += dx * c[a][0];
+    H01 += dx * c[a][1];
+    H10 += dy * c[a][0];
+    H11 += dy * c[a][1];
+  }
+  const T tr = H00 + H11;
+  const T S00 = lam * tr + T(2) * mu * H00;
+  const T S11 = lam * tr + T(2) * mu * H11;
+  const T S01 = mu * (H01 + H10);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    F[2 * a + 0] = s * (S00 * c[a][0] + S01 * c[a][1]);
+    F[2 * a + 1] = s * (S01 * c[a][0] + S11 * c[a][1]);
+  }
+  if (fix) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if ((fix >> i) & 1u) {
+        const int a = i / 2, cc = i % 2;
+        const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1];
+        F[i] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2) * U[i];
+      }
+  }
+  T en = T(0);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) en += U[i] * F[i];
+  return en;
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T elem_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
+                                        T *F) {
+  if constexpr (DIM == 3)
+    return tet_force<T>(X, U, lam, mu, es, fix, F);
+  else
+    return tri_force<T>(X, U, lam, mu, es, fix, F);
+}
+
+// Diagonal entries K_ii = s((lam+mu) c_ai^2 + mu |c_a|^2).
+template <typename T, int DIM>
+__device__ __forceinline__ void elem_diag(const T *X, T lam, T mu, T es, T *F) {
+  if constexpr (DIM == 3) {
+    const T e1x = X[3] - X[0], e1y = X[4] - X[1], e1z = X[5] - X[2];
+    const T e2x = X[6] - X[0], e2y = X[7] - X[1], e2z = X[8] - X[2];
+    const T e3x = X[9] - X[0], e3y = X[10] - X[1], e3z = X[11] - X[2];
+    T c[4][3];
+    c[1][0] = e2y * e3z - e2z * e3y;
+    c[1][1] = e2z * e3x - e2x * e3z;
+    c[1][2] = e2x * e3y - e2y * e3x;
+    c[2][0] = e3y * e1z - e3z * e1y;
+    c[2][1] = e3z * e1x - e3x * e1z;
+    c[2][2] = e3x * e1y - e3y * e1x;
+    c[3][0] = e1y * e2z - e1z * e2y;
+    c[3][1] = e1z * e2x - e1x * e2z;
+    c[3][2] = e1x * e2y - e1y * e2x;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) c[0][j] = -(c[1][j] + c[2][j] + c[3][j]);
+    const T det = e1x * c[1][0] + e1y * c[1][1] + e1z * c[1][2];
+    const T s = es / (T(6) * fabs(det));
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1] + c[a][2] * c[a][2];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) F[3 * a + cc] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2);
+    }
+  } else {
+    const T e1x = X[2] - X[0], e1y = X[3] - X[1];
+    const T e2x = X[4] - X[0], e2y = X[5] - X[1];
+    T c[3][2];
+    c[1][0] = e2y;
+    c[1][1] = -e2x;
+    c[2][0] = -e1y;
+    c[2][1] = e1x;
+    c[0][0] = -(c[1][0] + c[2][0]);
+    c[0][1] = -(c[1][1] + c[2][1]);
+    const T det = e1x * e2y - e1y * e2x;
+    const T s = es / (T(2) * fabs(det));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1];
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) F[2 * a + cc] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step
+// (p = z + beta p, q = cK K p + cM m p, p'q reduced to alpha).  MODE 2: diag(K).
+
+template <typename T>
+struct PatchArgs {
+  const PatchInfo *patches;
+  const uint32_t *elem_lid;
+  const int32_t *sh_gid;
+  const int32_t *sh_slot;
+  const uint16_t *node_end;
+  const uint16_t *inc;
+  const uint8_t *fixmask;
+  const int32_t *slot_start;
+  const T *coords;
+  const T *escale;
+  T lam, mu;
+  int np, p8, emax, pn;
+  int64_t n_owned;
+  const T *x;   // MODE 0: x; MODE 1: z
+  T *p;         // MODE 1: search direction (old in, new out for owned nodes)
+  T *y;         // owned-node output
+  T *partial;   // interface slots
+  OpShift<T> sh;
+  CgScalars *sc;
+  double *part;
+};
+
+template <typename T, int DIM, int MODE>
+__global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
+  constexpr int NPE = DIM + 1, NDE = DIM * NPE;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int b = blockIdx.x;
+  // XCD-aware: consecutive patches (neighbours on the Morton curve, sharing
+  // interface nodes) go to the same XCD and its L2.
+  const int pid = (b & 7) * a.p8 + (b >> 3);
+  if (pid >= a.np) return;
+  if (MODE == 1 && a.sc->done) return;
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.emax);
+  T *s_c = reinterpret_cast<T *>(lds + L.off_c);
+  T *s_x = reinterpret_cast<T *>(lds + L.off_x);
+  T *s_res = reinterpret_cast<T *>(lds + L.off_res);
+  uint16_t *s_end = reinterpret_cast<uint16_t *>(lds + L.off_end);
+  uint16_t *s_inc = reinterpret_cast<uint16_t *>(lds + L.off_inc);
+  uint8_t *s_fix = reinterpret_cast<uint8_t *>(lds + L.off_fix);
+  double *s_red = reinterpret_cast<double *>(lds + L.off_red);
+  int *s_flag = reinterpret_cast<int *>(lds + L.off_flag);
+
+  const PatchInfo P = a.patches[pid];
+  const int nloc = P.n_own + P.n_sh;
+  double acc_en = 0.0;
+  const T beta = MODE == 1 ? T(a.sc->beta) : T(0);
+  const T cK = MODE == 1 ? a.sh.cK : T(1);
+  const T cM = MODE == 1 ? a.sh.cM : T(0);
+
+  // Phase A: stage the patch's nodes in LDS.
+  for (int k = threadIdx.x; k < nloc; k += NT) {
+    const bool own = k < P.n_own;
+    const int64_t gid = own ? int64_t(P.own_start + k) : int64_t(a.sh_gid[P.sh_start + k - P.n_own]);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) s_c[k * DIM + c] = a.coords[gid * DIM + c];
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) s_x[k * DIM + c] = a.x[gid * DIM + c];
+    } else if constexpr (MODE == 1) {
+      T pn[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        pn[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
+        s_x[k * DIM + c] = pn[c];
+      }
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = pn[c];
+      }
+      if (a.sh.m) {
+        // mass term p'M p: owned nodes here, interface nodes by their first patch
+        const bool first = own || a.sh_slot[P.sh_start + k - P.n_own] ==
+                                      a.slot_start[gid - a.n_owned];
+        if (first) {
+#pragma unroll
+          for (int c = 0; c < DIM; ++c)
+            acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * double(pn[c]) * double(pn[c]);
+        }
+      }
+    }
+    s_fix[k] = a.fixmask[gid];
+    s_end[k] = a.node_end[P.pnode_start + k];
+  }
+  for (int j = threadIdx.x; j < P.n_elem * NPE; j += NT)
+    s_inc[j] = a.inc[int64_t(P.elem_start) * NPE + j];
+  __syncthreads();
+
+  // Phase B: one element per thread, forces into LDS.
+  for (int i = threadIdx.x; i < P.n_elem; i += NT) {
+    const uint32_t lid = a.elem_lid[P.elem_start + i];
+    T X[NDE], U[NDE];
+    unsigned fix = 0;
+#pragma unroll
+    for (int n = 0; n < NPE; ++n) {
+      const int l = int((lid >> (8 * n)) & 255u);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        X[n * DIM + c] = s_c[l * DIM + c];
+        if constexpr (MODE != 2) U[n * DIM + c] = s_x[l * DIM + c];
+      }
+      if constexpr (MODE != 2) fix |= unsigned(s_fix[l] & ((1 << DIM) - 1)) << (n * DIM);
+    }
+    const T es = a.escale ? a.escale[P.elem_start + i] : T(1);
+    T F[NDE];
+    if constexpr (MODE == 2) {
+      elem_diag<T, DIM>(X, a.lam, a.mu, es, F);
+    } else {
+      const T en = elem_force<T, DIM>(X, U, a.lam, a.mu, es, fix, F);
+      if constexpr (MODE == 1) acc_en += double(cK) * double(en);
+    }
+#pragma unroll
+    for (int j = 0; j < NDE; ++j) s_res[j * a.emax + i] = F[j];
+  }
+  __syncthreads();
+
+  // Phase C: fixed-order node sums; owned nodes final, interface to slots.
+  for (int k = threadIdx.x; k < nloc; k += NT) {
+    const int b0 = k ? int(s_end[k - 1]) : 0, b1 = int(s_end[k]);
+    T acc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+    for (int j = b0; j < b1; ++j) {
+      const int code = s_inc[j];
+      const int inst = code >> 2, corner = code & 3;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) acc[c] += s_res[(corner * DIM + c) * a.emax + inst];
+    }
+    if (k < P.n_own) {
+      const int64_t gid = P.own_start + k;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        T v = acc[c];
+        if constexpr (MODE == 1) {
+          v = cK * v;
+          if (a.sh.m) v += cM * a.sh.m[gid * DIM + c] * s_x[k * DIM + c];
+        }
+        a.y[gid * DIM + c] = v;
+      }
+    } else {
+      const int64_t slot = a.sh_slot[P.sh_start + k - P.n_own];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = MODE == 1 ? cK * acc[c] : acc[c];
+    }
+  }
+
+  if constexpr (MODE == 1) {
+    const double tot = block_sum(acc_en, s_red);
+    if (threadIdx.x == 0) store_agent(&a.part[pid], tot);
+    if (arrive_last(&a.sc->cnt_a, unsigned(a.np), s_flag)) {
+      double v = 0.0;
+      for (int i = threadIdx.x; i < a.np; i += NT) v += load_agent(&a.part[i]);
+      v = block_sum(v, s_red);
+      if (threadIdx.x == 0) {
+        CgScalars *sc = a.sc;
+        sc->pq = v;
+        const double alpha = sc->gamma / v;
+        if (!(v > 0.0) || !isfinite(alpha)) {
+          sc->alpha = 0.0;
+          sc->done = isfinite(v) ? 3 : 2;
+        } else {
+          sc->alpha = alpha;
+        }
+        __hip_atomic_store(&sc->cnt_a, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+// Interface gather: y[iface] = sum of slots (fixed order).
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_iface_gather(int64_t n_owned, int64_t n_iface,
+                                                     const int32_t *slot_start, const T *partial,
+                                                     T *y) {
+  const int64_t s = int64_t(blockIdx.x) * NT + threadIdx.x;
+  if (s >= n_iface) return;
+  T acc[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) acc[c] = T(0);
+  for (int j = slot_start[s]; j < slot_start[s + 1]; ++j)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] += partial[int64_t(j) * DIM + c];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) y[(n_owned + s) * DIM + c] = acc[c];
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector kernels.
+
+template <typename T>
+struct UpdArgs {
+  int64_t n_owned, n_active;
+  const int32_t *slot_start;
+  const T *partial;
+  const T *q;
+  T *p, *x, *r, *z;
+  const T *invD;
+  OpShift<T> sh;
+  CgScalars *sc;
+  double *part_a, *part_b;
+};
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_pcg_update(UpdArgs<T> a) {
+  __shared__ double s_red[NT / 64];
+  __shared__ int s_flag;
+  if (a.sc->done) return;
+  const T alpha = T(a.sc->alpha), beta = T(a.sc->beta);
+  double rz = 0.0, rr = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < a.n_active; i += stride) {
+    T qv[DIM], pv[DIM];
+    if (i < a.n_owned) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        qv[c] = a.q[i * DIM + c];
+        pv[c] = a.p[i * DIM + c];
+      }
+    } else {
+      const int64_t s = i - a.n_owned;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) qv[c] = T(0);
+      for (int j = a.slot_start[s]; j < a.slot_start[s + 1]; ++j)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) qv[c] += a.partial[int64_t(j) * DIM + c];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        pv[c] = a.z[i * DIM + c] + beta * a.p[i * DIM + c];
+        a.p[i * DIM + c] = pv[c];
+        if (a.sh.m) qv[c] += a.sh.cM * a.sh.m[i * DIM + c] * pv[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const int64_t o = i * DIM + c;
+      a.x[o] = a.x[o] + alpha * pv[c];
+      const T rv = a.r[o] - alpha * qv[c];
+      a.r[o] = rv;
+      const T zv = a.invD[o] * rv;
+      a.z[o] = zv;
+      rz += double(rv) * double(zv);
+      rr += double(rv) * double(rv);
+    }
+  }
+  rz = block_sum(rz, s_red);
+  rr = block_sum(rr, s_red);
+  if (threadIdx.x == 0) {
+    store_agent(&a.part_a[blockIdx.x], rz);
+    store_agent(&a.part_b[blockIdx.x], rr);
+  }
+  if (arrive_last(&a.sc->cnt_b, gridDim.x, &s_flag)) {
+    double g = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < int(gridDim.x); i += NT) {
+      g += load_agent(&a.part_a[i]);
+      q += load_agent(&a.part_b[i]);
+    }
+    g = block_sum(g, s_red);
+    q = block_sum(q, s_red);
+    if (threadIdx.x == 0) {
+      CgScalars *sc = a.sc;
+      sc->iter += 1;
+      sc->rr = q;
+      sc->scale = g;  // last gamma_new
+      const bool stop = sc->kind == 0 ? (g < sc->thresh) : (q <= sc->thresh);
+      if (stop) {
+        sc->done = 1;
+      } else {
+        sc->beta = g / sc->gamma;
+        sc->gamma = g;
+        if (!isfinite(g) || !isfinite(sc->beta)) sc->done = 2;
+      }
+      if (!sc->done && sc->iter >= sc->max_iter) sc->done = 4;
+      __hip_atomic_store(&sc->cnt_b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_init(int64_t n, const T *r, T *z, const T *invD,
+                                                 CgScalars *sc, double *part_a, double *part_b) {
+  __shared__ double s_red[NT / 64];
+  __shared__ int s_flag;
+  double rz = 0.0, rr = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+    const T rv = r[i];
+    const T zv = invD[i] * rv;
+    z[i] = zv;
+    rz += double(rv) * double(zv);
+    rr += double(rv) * double(rv);
+  }
+  rz = block_sum(rz, s_red);
+  rr = block_sum(rr, s_red);
+  if (threadIdx.x == 0) {
+    store_agent(&part_a[blockIdx.x], rz);
+    store_agent(&part_b[blockIdx.x], rr);
+  }
+  if (arrive_last(&sc->cnt_b, gridDim.x, &s_flag)) {
+    double g = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < int(gridDim.x); i += NT) {
+      g += load_agent(&part_a[i]);
+      q += load_agent(&part_b[i]);
+    }
+    g = block_sum(g, s_red);
+    q = block_sum(q, s_red);
+    if (threadIdx.x == 0) {
+      sc->gamma0 = g;
+      sc->gamma = g;
+      sc->scale = g;
+      sc->rr0 = q;
+      sc->rr = q;
+      sc->alpha = 0.0;
+      sc->beta = 0.0;
+      sc->iter = 0;
+      sc->thresh = sc->kind == 0 ? sc->eps * g : sc->eps * sc->eps * q;
+      sc->done = q == 0.0 ? 1 : (isfinite(g) ? 0 : 2);
+      __hip_atomic_store(&sc->cnt_b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ void k_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->gamma = sc->gamma0 = sc->alpha = sc->beta = sc->pq = sc->rr = sc->rr0 = 0.0;
+    sc->thresh = 0.0;
+    sc->eps = eps;
+    sc->kind = kind;
+    sc->done = 0;
+    sc->iter = 0;
+    sc->max_iter = max_iter;
+    sc->cnt_a = 0u;
+    sc->cnt_b = 0u;
+    sc->scale = 0.0;
+  }
+}
+
+// r = f - K x for the FP64 refinement loop; rr -> sc->rr, sqrt -> sc->scale.
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_residual64(int64_t n_owned, int64_t n_active,
+                                                   const int32_t *slot_start, const double *f,
+                                                   const double *y, const double *partial,
+                                                   double *r, CgScalars *sc, double *part) {
+  __shared__ double s_red[NT / 64];
+  __shared__ int s_flag;
+  double rr = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n_active; i += stride) {
+    double kv[DIM];
+    if (i < n_owned) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) kv[c] = y[i * DIM + c];
+    } else {
+      const int64_t s = i - n_owned;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) kv[c] = 0.0;
+      for (int j = slot_start[s]; j < slot_start[s + 1]; ++j)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) kv[c] += partial[int64_t(j) * DIM + c];
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const double rv = f[i * DIM + c] - kv[c];
+      r[i * DIM + c] = rv;
+      rr += rv * rv;
+    }
+  }
+  rr = block_sum(rr, s_red);
+  if (threadIdx.x == 0) store_agent(&part[blockIdx.x], rr);
+  if (arrive_last(&sc->cnt_b, gridDim.x, &s_flag)) {
+    double q = 0.0;
+    for (int i = threadIdx.x; i < int(gridDim.x); i += NT) q += load_agent(&part[i]);
+    q = block_sum(q, s_red);
+    if (threadIdx.x == 0) {
+      sc->rr = q;
+      sc->scale = sqrt(q);
+      __hip_atomic_store(&sc->cnt_b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_to_f32_scaled(int64_t n, const double *r64, float *r32,
+                                                      float *x32, const CgScalars *sc) {
+  const double s = sc->scale > 0.0 ? 1.0 / sc->scale : 0.0;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+    r32[i] = float(r64[i] * s);
+    x32[i] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_axpy_back(int64_t n, double *x64, const float *x32,
+                                                  const CgScalars *sc) {
+  const double s = sc->scale;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride)
+    x64[i] += s * double(x32[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_inv_diag(int64_t n, const T *diag, T cK, T cM, const T *m,
+                                                 T *invD) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+    const T v = cK * diag[i] + (m ? cM * m[i] : T(0));
+    invD[i] = v != T(0) ? T(1) / v : T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fill(T *p, int64_t n, T v) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(NT) void k_convert(const TA *a, TB *b, int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) b[i] = TB(a[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Per-element outputs (internal element order).
+
+template <typename T, int DIM>
+__device__ __forceinline__ void load_elem(const int32_t *elems, const T *coords, const T *u,
+                                          int64_t e, T *X, T *U) {
+  constexpr int NPE = DIM + 1;
+#pragma unroll
+  for (int n = 0; n < NPE; ++n) {
+    const int64_t g = elems[e * NPE + n];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      X[n * DIM + c] = coords[g * DIM + c];
+      U[n * DIM + c] = u[g * DIM + c];
+    }
+  }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_stress(int64_t ne, const int32_t *elems, const T *coords,
+                                               const T *escale, T lam, T mu, const T *u, T *sigma) {
+  constexpr int NDE = DIM * (DIM + 1), NV = 3 * (DIM - 1);
+  const int64_t e = int64_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= ne) return;
+  T X[NDE], U[NDE];
+  load_elem<T, DIM>(elems, coords, u, e, X, U);
+  const T es = escale ? escale[e] : T(1);
+  // sigma = S(H)/det with H = sum_a du_a (x) c_a (signed det: true gradients).
+  if constexpr (DIM == 3) {
+    const T e1x = X[3] - X[0], e1y = X[4] - X[1], e1z = X[5] - X[2];
+    const T e2x = X[6] - X[0], e2y = X[7] - X[1], e2z = X[8] - X[2];
+    const T e3x = X[9] - X[0], e3y = X[10] - X[1], e3z = X[11] - X[2];
+    T c[4][3];
+    c[1][0] = e2y * e3z - e2z * e3y;
+    c[1][1] = e2z * e3x - e2x * e3z;
+    c[1][2] = e2x * e3y - e2y * e3x;
+    c[2][0] = e3y * e1z - e3z * e1y;
+    c[2][1] = e3z * e1x - e3x * e1z;
+    c[2][2] = e3x * e1y - e3y * e1x;
+    c[3][0] = e1y * e2z - e1z * e2y;
+    c[3][1] = e1z * e2x - e1x * e2z;
+    c[3][2] = e1x * e2y - e1y * e2x;
+    const T det = e1x * c[1][0] + e1y * c[1][1] + e1z * c[1][2];
+    T H[3][3] = {};
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const T du = U[3 * a + i] - U[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) H[i][j] += du * c[a][j];
+      }
+    const T k = es / det;
+    const T tr = H[0][0] + H[1][1] + H[2][2];
+    T *o = sigma + e * NV;
+    o[0] = k * (lam * tr + T(2) * mu * H[0][0]);
+    o[1] = k * (lam * tr + T(2) * mu * H[1][1]);
+    o[2] = k * (lam * tr + T(2) * mu * H[2][2]);
+    o[3] = k * mu * (H[0][1] + H[1][0]);
+    o[4] = k * mu * (H[1][2] + H[2][1]);
+    o[5] = k * mu * (H[0][2] + H[2][0]);
+  } else {
+    const T e1x = X[2] - X[0], e1y = X[3] - X[1];
+    const T e2x = X[4] - X[0], e2y = X[5] - X[1];
+    T c[3][2];
+    c[1][0] = e2y;
+    c[1][1] = -e2x;
+    c[2][0] = -e1y;
+    c[2][1] = e1x;
+    const T det = e1x * e2y - e1y * e2x;
+    T H00 = T(0), H01 = T(0), H10 = T(0), H11 = T(0);
+#pragma unroll
+    for (int a = 1; a < 3; ++a) {
+      const T dx = U[2 * a] - U[0], dy = U[2 * a + 1] - U[1];
+      H00 += dx * c[a][0];
+      H01 += dx * c[a][1];
+      H10 += dy * c[a][0];
+      H11 += dy * c[a][1];
+    }
+    const T k = es / det;
+    const T tr = H00 + H11;
+    T *o = sigma + e * NV;
+    o[0] = k * (lam * tr + T(2) * mu * H00);
+    o[1] = k * (lam * tr + T(2) * mu * H11);
+    o[2] = k * mu * (H01 + H10);
+  }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_energy(int64_t ne, const int32_t *elems, const T *coords,
+                                               const uint8_t *fixmask, const T *escale, T lam,
+                                               T mu, const T *u, T *en) {
+  constexpr int NPE = DIM + 1, NDE = DIM * NPE;
+  const int64_t e = int64_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= ne) return;
+  T X[NDE], U[NDE], F[NDE];
+  load_elem<T, DIM>(elems, coords, u, e, X, U);
+  unsigned fix = 0;
+#pragma unroll
+  for (int n = 0; n < NPE; ++n)
+    fix |= unsigned(fixmask[elems[e * NPE + n]] & ((1 << DIM) - 1)) << (n * DIM);
+  const T es = escale ? escale[e] : T(1);
+  en[e] = elem_force<T, DIM>(X, U, lam, mu, es, fix, F);
+}
+
+// ---------------------------------------------------------------------------
+// Dynamics (GN22, femfunc.cpp:247-343) on assembled vectors.
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_dyn_predict(int64_t n, T *d, T *v, T *a, T dt, T c1, T c2,
+                                                    int zero_a) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+    const T av = a[i];
+    d[i] = d[i] + dt * v[i];
+    d[i] = d[i] + c1 * av;
+    v[i] = v[i] + c2 * av;
+    if (zero_a) a[i] = T(0);
+  }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_dyn_correct(int64_t n_owned, int64_t n_active,
+                                                    const int32_t *slot_start, const T *y,
+                                                    const T *partial, const T *f_file,
+                                                    const T *f_unit, const double *coef_file,
+                                                    const double *coef_unit, const int *step_base,
+                                                    int step_off, const T *invM, T *a, T *v,
+                                                    T *r_out, T c3) {
+  const int step = *step_base + step_off;
+  const T cf = T(coef_file[step]), cu = T(coef_unit[step]);
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n_active; i += stride) {
+    T kd[DIM];
+    if (i < n_owned) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) kd[c] = y[i * DIM + c];
+    } else {
+      const int64_t s = i - n_owned;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) kd[c] = T(0);
+      for (int j = slot_start[s]; j < slot_start[s + 1]; ++j)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) kd[c] += partial[int64_t(j) * DIM + c];
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const int64_t o = i * DIM + c;
+      const T r = cf * f_file[o] + cu * f_unit[o] - kd[c];
+      if (invM) {
+        const T av = r * invM[o];
+        a[o] = av;
+        v[o] = v[o] + c3 * av;
+      } else {
+        r_out[o] = r;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_dyn_implicit_post(int64_t n, const T *a, T *v, T *d, T c3,
+                                                          T c4) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
+    v[i] = v[i] + c3 * a[i];
+    d[i] = d[i] + c4 * a[i];
+  }
+}
+
+__global__ void k_set_int(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+inline int grid_for(int64_t n) {
+  int64_t g = (n + NT - 1) / NT;
+  if (g < 1) g = 1;
+  if (g > NODE_GRID_MAX) g = NODE_GRID_MAX;
+  return int(g);
+}
+
+template <typename T>
+PatchArgs<T> patch_args(const DevMesh &dm) {
+  PatchArgs<T> a{};
+  a.patches = dm.patches;
+  a.elem_lid = dm.elem_lid;
+  a.sh_gid = dm.sh_gid;
+  a.sh_slot = dm.sh_slot;
+  a.node_end = dm.node_end;
+  a.inc = dm.inc;
+  a.fixmask = dm.fixmask;
+  a.slot_start = dm.slot_start;
+  if constexpr (sizeof(T) == 8) {
+    a.coords = reinterpret_cast<const T *>(dm.coords64);
+    a.escale = reinterpret_cast<const T *>(dm.escale64);
+  } else {
+    a.coords = reinterpret_cast<const T *>(dm.coords32);
+    a.escale = reinterpret_cast<const T *>(dm.escale32);
+  }
+  a.lam = T(dm.lam);
+  a.mu = T(dm.mu);
+  a.np = dm.np;
+  a.p8 = (dm.np + 7) / 8;
+  a.emax = dm.emax;
+  a.pn = dm.pn;
+  a.n_owned = dm.n_owned;
+  return a;
+}
+
+template <typename T, int DIM, int MODE>
+void launch_patch(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.emax);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ebe_patch<T, DIM, MODE>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  const int grid = 8 * a.p8;
+  hipLaunchKernelGGL((k_ebe_patch<T, DIM, MODE>), dim3(grid), dim3(NT), L.total, s, a);
+}
+
+}  // namespace
+
+int lds_bytes(int dim, int elem_bytes, int pn, int emax) {
+  return lds_layout(dim, elem_bytes, pn, emax).total;
+}
+
+template <typename T>
+void launch_matvec_patch(const DevMesh &dm, const T *x, T *y, T *partial, hipStream_t s) {
+  PatchArgs<T> a = patch_args<T>(dm);
+  a.x = x;
+  a.y = y;
+  a.partial = partial;
+  if (dm.dim == 3)
+    launch_patch<T, 3, 0>(dm, a, s);
+  else
+    launch_patch<T, 2, 0>(dm, a, s);
+}
+
+template <typename T>
+void launch_matvec(const DevMesh &dm, const T *x, T *y, T *partial, hipStream_t s) {
+  launch_matvec_patch<T>(dm, x, y, partial, s);
+  if (dm.n_iface > 0) {
+    const int g = int((dm.n_iface + NT - 1) / NT);
+    if (dm.dim == 3)
+      hipLaunchKernelGGL((k_iface_gather<T, 3>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_iface,
+                         dm.slot_start, partial, y);
+    else
+      hipLaunchKernelGGL((k_iface_gather<T, 2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_iface,
+                         dm.slot_start, partial, y);
+  }
+}
+
+template <typename T>
+void launch_diag(const DevMesh &dm, T *d, T *partial, hipStream_t s) {
+  PatchArgs<T> a = patch_args<T>(dm);
+  a.y = d;
+  a.partial = partial;
+  if (dm.dim == 3)
+    launch_patch<T, 3, 2>(dm, a, s);
+  else
+    launch_patch<T, 2, 2>(dm, a, s);
+  if (dm.n_iface > 0) {
+    const int g = int((dm.n_iface + NT - 1) / NT);
+    if (dm.dim == 3)
+      hipLaunchKernelGGL((k_iface_gather<T, 3>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_iface,
+                         dm.slot_start, partial, d);
+    else
+      hipLaunchKernelGGL((k_iface_gather<T, 2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_iface,
+                         dm.slot_start, partial, d);
+  }
+}
+
+template <typename T>
+void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T *invD,
+                     hipStream_t s) {
+  const int64_t n = dm.nn * dm.dim;
+  hipLaunchKernelGGL((k_inv_diag<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, diag, cK, cM, m, invD);
+}
+
+template <typename T>
+void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScalars *sc,
+                     double *part, hipStream_t s) {
+  const int64_t n = dm.n_active * dm.dim;
+  const int g = grid_for(n);
+  hipLaunchKernelGGL((k_pcg_init<T>), dim3(g), dim3(NT), 0, s, n, r, z, invD, sc, part,
+                     part + NODE_GRID_MAX);
+}
+
+template <typename T>
+void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial, const T *invD,
+                     OpShift<T> sh, CgScalars *sc, double *part_a, double *part_b,
+                     hipStream_t s) {
+  PatchArgs<T> a = patch_args<T>(dm);
+  a.x = z;
+  a.p = p;
+  a.y = q;
+  a.partial = partial;
+  a.sh = sh;
+  a.sc = sc;
+  a.part = part_a;
+  if (dm.dim == 3)
+    launch_patch<T, 3, 1>(dm, a, s);
+  else
+    launch_patch<T, 2, 1>(dm, a, s);
+  UpdArgs<T> u{};
+  u.n_owned = dm.n_owned;
+  u.n_active = dm.n_active;
+  u.slot_start = dm.slot_start;
+  u.partial = partial;
+  u.q = q;
+  u.p = p;
+  u.x = x;
+  u.r = r;
+  u.z = z;
+  u.invD = invD;
+  u.sh = sh;
+  u.sc = sc;
+  u.part_a = part_b;
+  u.part_b = part_b + NODE_GRID_MAX;
+  const int g = grid_for(dm.n_active);
+  if (dm.dim == 3)
+    hipLaunchKernelGGL((k_pcg_update<T, 3>), dim3(g), dim3(NT), 0, s, u);
+  else
+    hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(g), dim3(NT), 0, s, u);
+}
+
+void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
+                       double *r, CgScalars *sc, double *part, hipStream_t s) {
+  const int g = grid_for(dm.n_active);
+  if (dm.dim == 3)
+    hipLaunchKernelGGL((k_residual64<3>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
+                       dm.slot_start, f, y, partial, r, sc, part);
+  else
+    hipLaunchKernelGGL((k_residual64<2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
+                       dm.slot_start, f, y, partial, r, sc, part);
+}
+
+void launch_to_f32_scaled(const DevMesh &dm, const double *r64, float *r32, float *x32,
+                          const CgScalars *sc, hipStream_t s) {
+  const int64_t n = dm.n_active * dm.dim;
+  hipLaunchKernelGGL(k_to_f32_scaled, dim3(grid_for(n)), dim3(NT), 0, s, n, r64, r32, x32, sc);
+}
+
+void launch_axpy_back(const DevMesh &dm, double *x64, const float *x32, const CgScalars *sc,
+                      hipStream_t s) {
+  const int64_t n = dm.n_active * dm.dim;
+  hipLaunchKernelGGL(k_axpy_back, dim3(grid_for(n)), dim3(NT), 0, s, n, x64, x32, sc);
+}
+
+template <typename T>
+void launch_fill(T *p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(NT), 0, s, p, n, v);
+}
+
+template <typename T>
+void launch_copy(const T *a, T *b, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_for(n)), dim3(NT), 0, s, a, b, n);
+}
+
+template <typename TA, typename TB>
+void launch_convert(const TA *a, TB *b, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_convert<TA, TB>), dim3(grid_for(n)), dim3(NT), 0, s, a, b, n);
+}
+
+template <typename T>
+void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s) {
+  const T *coords = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.coords64)
+                                   : reinterpret_cast<const T *>(dm.coords32);
+  const T *es = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.escale64)
+                               : reinterpret_cast<const T *>(dm.escale32);
+  const int g = int((dm.ne + NT - 1) / NT);
+  if (dm.dim == 3)
+    hipLaunchKernelGGL((k_stress<T, 3>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords, es,
+                       T(dm.lam), T(dm.mu), u, sigma);
+  else
+    hipLaunchKernelGGL((k_stress<T, 2>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords, es,
+                       T(dm.lam), T(dm.mu), u, sigma);
+}
+
+template <typename T>
+void launch_energy(const DevMesh &dm, const T *u, T *en, hipStream_t s) {
+  const T *coords = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.coords64)
+                                   : reinterpret_cast<const T *>(dm.coords32);
+  const T *es = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.escale64)
+                               : reinterpret_cast<const T *>(dm.escale32);
+  const int g = int((dm.ne + NT - 1) / NT);
+  if (dm.dim == 3)
+    hipLaunchKernelGGL((k_energy<T, 3>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords,
+                       dm.fixmask, es, T(dm.lam), T(dm.mu), u, en);
+  else
+    hipLaunchKernelGGL((k_energy<T, 2>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords,
+                       dm.fixmask, es, T(dm.lam), T(dm.mu), u, en);
+}
+
+template <typename T>
+void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
+                        hipStream_t s) {
+  const int64_t n = dm.n_active * dm.dim;
+  hipLaunchKernelGGL((k_dyn_predict<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, d, v, a, T(c.dt),
+                     T(c.c1), T(c.c2), int(zero_a));
+}
+
+template <typename T>
+void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T *f_file,
+                        const T *f_unit, const double *coef_file, const double *coef_unit,
+                        const int *step_base, int step_off, const T *invM, T *a, T *v, T *r_out,
+                        DynCoeffs c, hipStream_t s) {
+  const int g = grid_for(dm.n_active);
+  if (dm.dim == 3)
+    hipLaunchKernelGGL((k_dyn_correct<T, 3>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
+                       dm.slot_start, y, partial, f_file, f_unit, coef_file, coef_unit, step_base,
+                       step_off, invM, a, v, r_out, T(c.c3));
+  else
+    hipLaunchKernelGGL((k_dyn_correct<T, 2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
+                       dm.slot_start, y, partial, f_file, f_unit, coef_file, coef_unit, step_base,
+                       step_off, invM, a, v, r_out, T(c.c3));
+}
+
+template <typename T>
+void launch_dyn_implicit_post(const DevMesh &dm, const T *a, T *v, T *d, DynCoeffs c,
+                              hipStream_t s) {
+  const int64_t n = dm.n_active * dm.dim;
+  hipLaunchKernelGGL((k_dyn_implicit_post<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, a, v, d,
+                     T(c.c3), T(c.c4));
+}
+
+void launch_set_int(int *p, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
+}
+
+void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, kind, eps, max_iter);
+}
+
+#define FEMGPU_INST(T)                                                                          \
+  template void launch_matvec<T>(const DevMesh &, const T *, T *, T *, hipStream_t);           \
+  template void launch_matvec_patch<T>(const DevMesh &, const T *, T *, T *, hipStream_t);     \
+  template void launch_diag<T>(const DevMesh &, T *, T *, hipStream_t);                        \
+  template void launch_inv_diag<T>(const DevMesh &, const T *, T, T, const T *, T *,           \
+                                   hipStream_t);                                               \
+  template void launch_pcg_init<T>(const DevMesh &, const T *, T *, const T *, CgScalars *,    \
+                                   double *, hipStream_t);                                     \
+  template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, T *, const T *,   \
+                                   OpShift<T>, CgScalars *, double *, double *, hipStream_t);  \
+  template void launch_fill<T>(T *, int64_t, T, hipStream_t);                                  \
+  template void launch_copy<T>(const T *, T *, int64_t, hipStream_t);                          \
+  template void launch_stress<T>(const DevMesh &, const T *, T *, hipStream_t);                \
+  template void launch_energy<T>(const DevMesh &, const T *, T *, hipStream_t);                \
+  template void launch_dyn_predict<T>(const DevMesh &, T *, T *, T *, DynCoeffs, bool,         \
+                                      hipStream_t);                                            \
+  template void launch_dyn_correct<T>(const DevMesh &, const T *, const T *, const T *,        \
+                                      const T *, const double *, const double *, const int *,  \
+                                      int, const T *, T *, T *, T *, DynCoeffs, hipStream_t);  \
+  template void launch_dyn_implicit_post<T>(const DevMesh &, const T *, T *, T *, DynCoeffs,   \
+                                            hipStream_t);
+
+FEMGPU_INST(double)
+FEMGPU_INST(float)
+template void launch_convert<double, float>(const double *, float *, int64_t, hipStream_t);
+template void launch_convert<float, double>(const float *, double *, int64_t, hipStream_t);
+template void launch_fill<int>(int *, int64_t, int, hipStream_t);
+
+}  // namespace femgpu

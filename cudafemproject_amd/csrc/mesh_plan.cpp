@@ -1,0 +1,257 @@
+// mesh_plan.cpp -- host-side preparation of the device mesh layout.
+//
+// Replaces the reference's per-iteration sort-based assembly set-up
+// (genMask + sort + reduce_by_key, fem_utils/src/cpu_fem_utils.cpp:182-219 and
+// fem_utils/src/cu/cuda_fem_utils.cpp:178-216): instead of sorting all
+// element-local entries by DOF every PCG iteration, the element -> patch and
+// node -> owner maps are built once here, and the kernels assemble in LDS and
+// through fixed-order interface sums.  See femgpu_internal.h for the layout.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "femgpu_internal.h"
+
+namespace femgpu {
+
+namespace {
+
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1fffffULL;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+uint64_t spread2(uint64_t v) {  // 31 bits -> every second bit
+  v &= 0x7fffffffULL;
+  v = (v | v << 16) & 0x0000ffff0000ffffULL;
+  v = (v | v << 8) & 0x00ff00ff00ff00ffULL;
+  v = (v | v << 4) & 0x0f0f0f0f0f0f0f0fULL;
+  v = (v | v << 2) & 0x3333333333333333ULL;
+  v = (v | v << 1) & 0x5555555555555555ULL;
+  return v;
+}
+
+}  // namespace
+
+std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
+                       const int32_t *elems, int emax, MeshPlan &plan) {
+  if (dim != 2 && dim != 3) return "dim must be 2 or 3";
+  if (nn <= 0 || ne <= 0) return "empty mesh";
+  if (nn >= (int64_t(1) << 31) / 3 || ne >= (int64_t(1) << 31) / 12)
+    return "mesh too large for int32 DOF indexing";
+  if (emax < 16 || emax > EMAX_LIMIT) return "elements per patch out of range";
+  const int npe = dim + 1;
+  plan = MeshPlan();
+  plan.dim = dim;
+  plan.npe = npe;
+  plan.nde = dim * npe;
+  plan.nn = nn;
+  plan.ne = ne;
+  plan.emax = emax;
+
+  for (int64_t e = 0; e < ne; ++e) {
+    for (int a = 0; a < npe; ++a) {
+      int32_t v = elems[e * npe + a];
+      if (v < 0 || v >= nn) return "element node index out of range";
+      for (int b = 0; b < a; ++b)
+        if (elems[e * npe + b] == v) return "element with a repeated node";
+    }
+  }
+
+  // 1. Morton order of element centroids.
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int c = 0; c < dim; ++c) lo[c] = hi[c] = nodes[c];
+  for (int64_t i = 0; i < nn; ++i)
+    for (int c = 0; c < dim; ++c) {
+      double v = nodes[i * dim + c];
+      if (!std::isfinite(v)) return "non-finite node coordinate";
+      lo[c] = std::min(lo[c], v);
+      hi[c] = std::max(hi[c], v);
+    }
+  const int bits = dim == 3 ? 21 : 31;
+  const double qmax = double((uint64_t(1) << bits) - 1);
+  std::vector<std::pair<uint64_t, int32_t>> key(ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    uint64_t q[3] = {0, 0, 0};
+    for (int c = 0; c < dim; ++c) {
+      double s = 0;
+      for (int a = 0; a < npe; ++a) s += nodes[int64_t(elems[e * npe + a]) * dim + c];
+      s /= npe;
+      double span = hi[c] - lo[c];
+      double t = span > 0 ? (s - lo[c]) / span : 0.0;
+      q[c] = uint64_t(std::min(qmax, std::max(0.0, t * qmax)));
+    }
+    uint64_t k = dim == 3 ? (spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2)
+                          : (spread2(q[0]) | spread2(q[1]) << 1);
+    key[e] = {k, int32_t(e)};
+  }
+  std::sort(key.begin(), key.end());
+  plan.elem_perm.resize(ne);
+  for (int64_t i = 0; i < ne; ++i) plan.elem_perm[i] = key[i].second;
+  key.clear();
+  key.shrink_to_fit();
+
+  // 2. Greedy patches over the sorted elements.
+  std::vector<int32_t> stamp(nn, -1);
+  std::vector<int32_t> pstart;  // element start per patch
+  {
+    int32_t pid = 0, cur_nodes = 0, cur_elems = 0;
+    pstart.push_back(0);
+    for (int64_t i = 0; i < ne; ++i) {
+      const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
+      int add = 0;
+      for (int a = 0; a < npe; ++a) add += stamp[en[a]] != pid;
+      if (cur_elems > 0 && (cur_elems + 1 > emax || cur_nodes + add > PN_MAX)) {
+        ++pid;
+        pstart.push_back(int32_t(i));
+        cur_nodes = 0;
+        cur_elems = 0;
+        add = npe;
+      }
+      for (int a = 0; a < npe; ++a)
+        if (stamp[en[a]] != pid) {
+          stamp[en[a]] = pid;
+          ++cur_nodes;
+        }
+      ++cur_elems;
+    }
+    pstart.push_back(int32_t(ne));
+  }
+  const int32_t np = int32_t(pstart.size()) - 1;
+
+  // 3. Touch counts: how many patches reference each node.
+  std::vector<int32_t> touches(nn, 0);
+  std::fill(stamp.begin(), stamp.end(), -1);
+  for (int32_t p = 0; p < np; ++p)
+    for (int32_t i = pstart[p]; i < pstart[p + 1]; ++i) {
+      const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
+      for (int a = 0; a < npe; ++a)
+        if (stamp[en[a]] != p) {
+          stamp[en[a]] = p;
+          ++touches[en[a]];
+        }
+    }
+
+  // 4. Internal node numbering: owned runs per patch, then interface, then isolated.
+  std::vector<int32_t> inv(nn, -1);
+  int32_t next = 0;
+  for (int32_t p = 0; p < np; ++p)
+    for (int32_t i = pstart[p]; i < pstart[p + 1]; ++i) {
+      const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
+      for (int a = 0; a < npe; ++a)
+        if (touches[en[a]] == 1 && inv[en[a]] < 0) inv[en[a]] = next++;
+    }
+  plan.n_owned = next;
+  for (int32_t p = 0; p < np; ++p)
+    for (int32_t i = pstart[p]; i < pstart[p + 1]; ++i) {
+      const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
+      for (int a = 0; a < npe; ++a)
+        if (touches[en[a]] > 1 && inv[en[a]] < 0) inv[en[a]] = next++;
+    }
+  plan.n_iface = next - plan.n_owned;
+  for (int64_t v = 0; v < nn; ++v)
+    if (inv[v] < 0) inv[v] = next++;
+  plan.n_isolated = nn - plan.n_owned - plan.n_iface;
+  plan.node_inv = inv;
+  plan.node_perm.assign(nn, 0);
+  for (int64_t v = 0; v < nn; ++v) plan.node_perm[inv[v]] = int32_t(v);
+
+  // 5. Interface slots (contiguous per node, patch order).
+  plan.slot_start.assign(plan.n_iface + 1, 0);
+  for (int64_t j = 0; j < plan.n_iface; ++j)
+    plan.slot_start[j + 1] = plan.slot_start[j] + touches[plan.node_perm[plan.n_owned + j]];
+  plan.n_slots = plan.slot_start[plan.n_iface];
+  std::vector<int32_t> slot_next(plan.slot_start.begin(), plan.slot_start.end() - 1);
+
+  // 6. Per-patch local numbering, packed ids and incidence lists.
+  plan.patches.resize(np);
+  plan.elem_lid.assign(ne, 0);
+  plan.inc.assign(size_t(ne) * npe, 0);
+  plan.elems_int.assign(size_t(ne) * npe, 0);
+  std::vector<int32_t> local(nn, -1);
+  std::vector<int32_t> loc_nodes;
+  std::vector<int32_t> cnt;
+  for (int32_t p = 0; p < np; ++p) {
+    PatchInfo &P = plan.patches[p];
+    P.elem_start = pstart[p];
+    P.n_elem = pstart[p + 1] - pstart[p];
+    P.sh_start = int32_t(plan.sh_gid.size());
+    P.pnode_start = int32_t(plan.node_end.size());
+    P.pad = 0;
+    // owned nodes: internal ids of this patch are one contiguous run
+    int32_t omin = INT32_MAX, omax = -1, n_own = 0;
+    loc_nodes.clear();
+    for (int32_t i = pstart[p]; i < pstart[p + 1]; ++i) {
+      const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
+      for (int a = 0; a < npe; ++a) {
+        int32_t v = en[a];
+        if (local[v] != -1) continue;
+        local[v] = -2;  // seen
+        loc_nodes.push_back(v);
+        if (touches[v] == 1) {
+          omin = std::min(omin, inv[v]);
+          omax = std::max(omax, inv[v]);
+          ++n_own;
+        }
+      }
+    }
+    if (n_own > 0 && omax - omin + 1 != n_own) return "internal error: owned run not contiguous";
+    P.own_start = n_own > 0 ? omin : 0;
+    P.n_own = n_own;
+    int32_t n_sh = 0;
+    for (int32_t v : loc_nodes) {
+      if (touches[v] == 1) {
+        local[v] = inv[v] - P.own_start;
+      } else {
+        local[v] = n_own + n_sh++;
+        plan.sh_gid.push_back(inv[v]);
+        int64_t j = inv[v] - plan.n_owned;
+        plan.sh_slot.push_back(slot_next[j]++);
+      }
+    }
+    P.n_sh = n_sh;
+    const int32_t nloc = n_own + n_sh;
+    if (nloc > PN_MAX) return "internal error: patch node overflow";
+    plan.pn_used = std::max(plan.pn_used, int(nloc));
+    plan.emax_used = std::max(plan.emax_used, int(P.n_elem));
+    // packed ids + counts for the incidence lists
+    cnt.assign(nloc, 0);
+    for (int32_t i = pstart[p]; i < pstart[p + 1]; ++i) {
+      const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
+      uint32_t packed = 0;
+      for (int a = 0; a < npe; ++a) {
+        int32_t l = local[en[a]];
+        packed |= uint32_t(l) << (8 * a);
+        ++cnt[l];
+        plan.elems_int[size_t(i) * npe + a] = inv[en[a]];
+      }
+      plan.elem_lid[i] = packed;
+    }
+    // end offsets, then fill (entries sorted by instance, then corner)
+    std::vector<int32_t> pos(nloc);
+    int32_t run = 0;
+    for (int32_t k = 0; k < nloc; ++k) {
+      pos[k] = run;
+      run += cnt[k];
+      plan.node_end.push_back(uint16_t(run));
+    }
+    uint16_t *seg = &plan.inc[size_t(P.elem_start) * npe];
+    for (int32_t i = 0; i < P.n_elem; ++i) {
+      uint32_t packed = plan.elem_lid[P.elem_start + i];
+      for (int a = 0; a < npe; ++a) {
+        int32_t l = int32_t((packed >> (8 * a)) & 255u);
+        seg[pos[l]++] = uint16_t(i * 4 + a);
+      }
+    }
+    for (int32_t v : loc_nodes) local[v] = -1;
+  }
+  return "";
+}
+
+}  // namespace femgpu

@@ -1,0 +1,210 @@
+/*
+ * femgpu.h -- C-ABI of the MI355X-native EbE linear-elasticity solve path.
+ *
+ * This is the drop-in boundary for the reference's solver path
+ * (moarseniy/CudaFEMproject).  The reference exposes that path as C++
+ * functions taking its own `Matrix`, `ElementsData` and `dataKeeper` objects:
+ *
+ *   femfunc/src/femfunc.h:20   gpuCalculateFEM_EbE_vec(DEVICE_NAME, dataKeeper&, bool)
+ *   femfunc/src/femfunc.h:21   gpuPCG_EbE_vec(Matrix&, DEVICE_NAME, dataKeeper&, ElementsData&, bool, float, bool)
+ *   femfunc/src/femfunc.h:22   gpuCalculateFEM_DYN(DEVICE_NAME, dataKeeper&, bool)
+ *   fem_utils/include/fem_utils_cpu.h:24-148  ElementsData (the element loops)
+ *   matrix_pack/include/matrix_pack_cpu.h:31-121  Matrix (batched local matrices)
+ *
+ * Each entry point below names the reference interface it replaces.  All
+ * signatures use plain pointers and sizes; host arrays are caller-owned and
+ * copied in; results are written to caller-owned host arrays.  One context
+ * per host thread; every call on a context is issued on the context's HIP
+ * stream (see femgpu_set_stream).  Every function returns an int status
+ * (FEMGPU_OK == 0, negative on error) and never exits the process, unlike the
+ * reference (Tools/src/Tools.cpp:27-33, femfunc.cpp:107-110).
+ */
+#ifndef FEMGPU_H
+#define FEMGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct femgpu_ctx femgpu_ctx;
+
+/* Status codes. */
+enum {
+  FEMGPU_OK = 0,
+  FEMGPU_E_ARG = -1,        /* bad argument                                  */
+  FEMGPU_E_HIP = -2,        /* HIP runtime error (message in last_error)     */
+  FEMGPU_E_NONFINITE = -3,  /* NaN/Inf in the iteration (femfunc.cpp:107)    */
+  FEMGPU_E_MAXIT = -4,      /* iteration cap reached before the tolerance    */
+  FEMGPU_E_BREAKDOWN = -5,  /* p'Kp <= 0 (indefinite or singular operator)   */
+  FEMGPU_E_BAD_MESH = -6,   /* degenerate element, bad index, face not on elem */
+  FEMGPU_E_STATE = -7,      /* call order (e.g. solve before set_mesh)       */
+  FEMGPU_E_NOMEM = -8,      /* device allocation failed                      */
+  FEMGPU_E_UNSUPPORTED = -9 /* input the reference only handles by accident  */
+};
+
+/* Arithmetic of a solve. */
+enum {
+  FEMGPU_FP64 = 0,   /* FP64 vectors, FP64 EbE matvec                        */
+  FEMGPU_FP32 = 1,   /* FP32 vectors and matvec, FP64 dot accumulation       */
+  FEMGPU_MIXED = 2   /* FP32 PCG inside FP64 iterative refinement            */
+};
+
+/* Stopping rules. */
+enum {
+  FEMGPU_TOL_REF_GAMMA = 0, /* gamma_new < tol*gamma0, gamma = r'M^-1 r      */
+                            /* (the reference's rule, femfunc.cpp:92)         */
+  FEMGPU_TOL_REL_RES = 1    /* ||f - K u||_2 <= tol * ||f||_2                */
+};
+
+/* Load conventions for pressure faces and nodal loads. */
+enum {
+  FEMGPU_LOADS_REFERENCE = 0, /* the reference's CPU path bit for bit in
+                                 meaning: F = c*p*len*n assigned per element
+                                 slot, 3D len = |a x b|^2/2 with c = -1/12,
+                                 component j dropped when (code & j), nodal
+                                 loads gathered per element (cpu_fem_utils.cpp:
+                                 564-596)                                    */
+  FEMGPU_LOADS_PHYSICAL = 1   /* F = -p*A/npf*n per face node (A = face
+                                 length/area), summed over faces, duplicate
+                                 faces once, zero on constrained DOFs; nodal
+                                 loads added once                            */
+};
+
+/* Mass lumping conventions for dynamics. */
+enum {
+  FEMGPU_MASS_REFERENCE = 0,  /* 2D rho*A/3, 3D rho*V/12 per DOF
+                                 (cpu_fem_utils.cpp:598-734)                 */
+  FEMGPU_MASS_PHYSICAL = 1    /* 2D rho*A/3, 3D rho*V/4 per DOF              */
+};
+
+/* --- context -------------------------------------------------------------
+ * Replaces: the static cuBLAS handle and CUDA_Matrix::_initCUDA
+ * (matrix_pack/src/cu/cuda_matrix.cpp:3-8, fem/src/main.cpp:18-20) and the
+ * device choice DEVICE_NAME (matrix_pack_cpu.h:19-23). */
+int femgpu_create(femgpu_ctx **ctx, int device_ordinal);
+void femgpu_destroy(femgpu_ctx *ctx);
+const char *femgpu_last_error(const femgpu_ctx *ctx);
+/* Use a caller-created hipStream_t (passed as void*); NULL = own stream. */
+int femgpu_set_stream(femgpu_ctx *ctx, void *hip_stream);
+/* Synchronise the context's stream. */
+int femgpu_sync(femgpu_ctx *ctx);
+
+/* --- problem definition ---------------------------------------------------
+ * Replaces: the dataKeeper readers (DataKeeper/src/datakeeper.cpp:116-213)
+ * and the ElementsData constructor's host->device copies
+ * (fem_utils/src/cu/cuda_fem_utils.cpp:13-47).  Index arrays are int32
+ * (the reference keeps them as float, datakeeper.cpp:124-137). */
+
+/* nodes: n_nodes x dim row-major; elems: n_elems x (dim+1) 0-based.
+ * Builds the device-resident mesh (element patches, node renumbering);
+ * resets material, constraints and loads. */
+int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nodes,
+                    int64_t n_elems, const int32_t *elems);
+/* Isotropic material (datakeeper.cpp:96-114, CreateMatrixD).  elem_scale:
+ * optional per-element modulus factor (n_elems, user element order). */
+int femgpu_set_material(femgpu_ctx *ctx, double E, double nu, const double *elem_scale);
+/* Constraint lines "node code" (datakeeper.cpp:139-171); code decoded
+ * bitwise as there: bit0 -> x, bit1 -> y, (code & 3) -> z in 3D. */
+int femgpu_set_constraints(femgpu_ctx *ctx, int64_t n, const int32_t *node,
+                           const int32_t *code);
+/* Pressure faces (stress.txt, datakeeper.cpp:187-213): face_nodes n x dim,
+ * adj_elem n, normal n x dim (normalised here), pressure n. */
+int femgpu_set_pressure_faces(femgpu_ctx *ctx, int64_t n, const int32_t *face_nodes,
+                              const int32_t *adj_elem, const double *normal,
+                              const double *pressure);
+/* Nodal loads (loads.txt, datakeeper.cpp:173-185): vec n x dim. */
+int femgpu_set_nodal_loads(femgpu_ctx *ctx, int64_t n, const int32_t *node, const double *vec);
+int femgpu_set_load_convention(femgpu_ctx *ctx, int convention);
+/* Optional explicit global force vector (n_nodes x dim) added to the RHS
+ * built from faces/loads (body forces, extra load cases).  NULL clears it. */
+int femgpu_set_extra_rhs(femgpu_ctx *ctx, const double *f);
+
+/* --- operators (test and integration hooks) -------------------------------
+ * Replaces: Matrix::bmm of the per-element K_e with the element vectors plus
+ * reductionWithMask (cpu_matrix.cpp:74-95, cpu_fem_utils.cpp:437-459): here
+ * y = K x is one on-the-fly EbE product into assembled global vectors.
+ * x, y: n_nodes x dim host arrays in user node order. */
+int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y);
+/* Assembled diagonal of K (Matrix::getDiagonal + reductionWithMask,
+ * cpu_matrix.cpp:296-305; femfunc.cpp:10, :41). */
+int femgpu_diag(femgpu_ctx *ctx, double *d);
+/* Assembled RHS at time t with wavelet scale (ElementsData::initFlocals /
+ * calcFlocals + reductionWithMask, cpu_fem_utils.cpp:522-596). */
+int femgpu_rhs(femgpu_ctx *ctx, double *f);
+/* Repeat the standalone FP32/FP64 EbE matvec `reps` times on device-resident
+ * internal vectors, enqueued on the context stream without synchronising
+ * (for timing with events on that stream). */
+int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps);
+
+/* --- static solve -----------------------------------------------------------
+ * Replaces: gpuCalculateFEM_EbE_vec + gpuPCG_EbE_vec (femfunc.cpp:3-128). */
+typedef struct {
+  int precision;      /* FEMGPU_FP64 / FEMGPU_FP32 / FEMGPU_MIXED             */
+  int tol_kind;       /* FEMGPU_TOL_REF_GAMMA / FEMGPU_TOL_REL_RES            */
+  double tol;         /* 1e-8 in the reference's static path (femfunc.cpp:20) */
+  int max_iter;       /* total PCG iteration cap; <= 0: 1,000,000             */
+  double inner_tol;   /* MIXED: relative residual reduction per FP32 solve    */
+  int check_every;    /* iterations between host polls of the stop flag       */
+} femgpu_solve_opts;
+
+typedef struct {
+  int status;          /* FEMGPU_OK or an error code                          */
+  int iterations;      /* PCG iterations (summed over inner solves in MIXED)  */
+  int outer_iterations;/* MIXED: refinement steps                             */
+  double rel_res;      /* true ||f - K u|| / ||f|| recomputed in FP64         */
+  double gamma_ratio;  /* final gamma/gamma0 of the (last) PCG                */
+  double solve_ms;     /* device time of the solve, setup excluded            */
+} femgpu_stats;
+
+int femgpu_default_solve_opts(femgpu_solve_opts *opts);
+/* u_out: n_nodes x dim (user order); may be NULL (solution stays on device). */
+int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
+                        femgpu_stats *stats);
+
+/* --- per-element results ----------------------------------------------------
+ * No counterpart in the reference's live code; the golden fixtures hold
+ * sigma = D B u_e per element (testing/<task>_test/results.vtk, TENSORS stress).
+ * u: n_nodes x dim; sigma_out: n_elems x 3(dim-1) Voigt (xx,yy,xy | xx,yy,zz,
+ * xy,yz,xz), user element order. */
+int femgpu_stress(femgpu_ctx *ctx, const double *u, double *sigma_out);
+/* Element strain energies e_e = u_e' K_e u_e (n_elems). */
+int femgpu_element_energy(femgpu_ctx *ctx, const double *u, double *e_out);
+
+/* --- dynamics ---------------------------------------------------------------
+ * Replaces: gpuCalculateFEM_DYN (femfunc.cpp:130-381): GN22 with beta2 == 0
+ * (explicit central difference, lumped mass, solveDiagSystem) or beta2 != 0
+ * (implicit, nested PCG per step).  Wavelet: fem_utils.cpp:53-82. */
+enum { FEMGPU_WAVELET_NONE = 0, FEMGPU_WAVELET_RICKER = 1, FEMGPU_WAVELET_BERLAGE = 2 };
+enum {
+  FEMGPU_IMPLICIT_REFERENCE = 0, /* solve K a = F - K d, preconditioned by
+                                    cK diag(K) + diag(M) (femfunc.cpp:204-212,302) */
+  FEMGPU_IMPLICIT_PHYSICAL = 1   /* solve (M + beta2 dt^2/2 K) a = F - K d     */
+};
+typedef struct {
+  double rho, beta1, beta2, dt, endtime;  /* endtime < 0: one step (relax stub) */
+  int wavelet_type;
+  double wavelet_freq, wavelet_timeshift, wavelet_ampl;
+  int precision;        /* FEMGPU_FP64 or FEMGPU_FP32 state                   */
+  int mass_convention;  /* FEMGPU_MASS_REFERENCE / FEMGPU_MASS_PHYSICAL        */
+  int implicit_mode;    /* FEMGPU_IMPLICIT_REFERENCE / FEMGPU_IMPLICIT_PHYSICAL */
+  double pcg_eps;       /* implicit inner PCG eps (1e-4 in femfunc.cpp:153)    */
+  int steps_per_graph;  /* explicit: steps captured per HIP graph (0: 64)      */
+} femgpu_dyn_opts;
+
+typedef struct {
+  int status;
+  int steps;
+  int pcg_iterations;  /* implicit: summed over steps                         */
+  double run_ms;       /* device time of the time loop                        */
+} femgpu_dyn_stats;
+
+int femgpu_default_dyn_opts(femgpu_dyn_opts *opts);
+int femgpu_dyn_run(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, double *u_out,
+                   femgpu_dyn_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

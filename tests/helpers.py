@@ -1,0 +1,56 @@
+"""Shared helpers of the test-suite (fixtures, dense reference assembly)."""
+import os
+
+import numpy as np
+
+from conftest import GOLDEN
+from cudafemproject_amd.problem import read_prepared
+
+FIXTURES = {"test_rect_pcg": (2.0e7, 0.25), "9task_3": (2.0e7, 0.25)}
+
+
+def read_vtk_field(path, name):
+    """Numbers following a 'VECTORS name' / 'TENSORS name' / 'SCALARS name' header
+    of a legacy ASCII VTK file, up to the next blank line."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    for i, ln in enumerate(lines):
+        parts = ln.split()
+        if len(parts) >= 2 and parts[0] in ("VECTORS", "TENSORS", "SCALARS") and parts[1] == name:
+            j = i + 1
+            if parts[0] == "SCALARS" and lines[j].startswith("LOOKUP_TABLE"):
+                j += 1
+            vals = []
+            while j < len(lines) and lines[j].strip():
+                vals.extend(float(v) for v in lines[j].split())
+                j += 1
+            return np.array(vals)
+    raise KeyError(name)
+
+
+def fixture_problem(name):
+    E, nu = FIXTURES[name]
+    return read_prepared(os.path.join(GOLDEN, name), 2, E=E, nu=nu)
+
+
+def fixture_displacements(name):
+    v = read_vtk_field(os.path.join(GOLDEN, name, "results.vtk"), "displacements")
+    return v.reshape(-1, 3)[:, :2]
+
+
+def fixture_stress(name):
+    """Per-element (sxx, syy, sxy) from the fixture's TENSORS stress."""
+    t = read_vtk_field(os.path.join(GOLDEN, name, "results.vtk"), "stress").reshape(-1, 3, 3)
+    return np.stack([t[:, 0, 0], t[:, 1, 1], t[:, 0, 1]], 1)
+
+
+def dense_K(pb):
+    """Assembled constrained stiffness from the FP64 oracle, column by column."""
+    import oracle as O
+    n = pb.n_dof
+    K = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        K[:, j] = O.matvec(pb, e.reshape(pb.n_nodes, pb.dim)).ravel()
+    return K

@@ -1,0 +1,138 @@
+"""GPU parity: the HIP path against the oracle (FP64/FP32 restatement of the
+reference's CPU algorithm) through the C-ABI.  Tolerances are stated per test."""
+import numpy as np
+import pytest
+
+import cudafemproject_amd as F
+import oracle as O
+from cudafemproject_amd.problem import (box_tet, cube_problem, plate_problem, column_problem,
+                                        Problem, cantilever_problem)
+from helpers import fixture_problem, fixture_displacements, fixture_stress
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _problems():
+    p2 = plate_problem(12, 7, E=2e7, nu=0.25)
+    p3 = cube_problem(5, convention=F.LOADS_REFERENCE)
+    p3s = cube_problem(4, convention=F.LOADS_REFERENCE)
+    p3s.elem_scale = np.random.default_rng(3).uniform(0.05, 1.0, p3s.n_elems) ** 3
+    return {"plate": p2, "cube": p3, "cube_scaled": p3s,
+            "rect_pcg": fixture_problem("test_rect_pcg"), "9task_3": fixture_problem("9task_3")}
+
+
+PROBLEMS = _problems()
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_matvec_fp64_vs_oracle(ctx, name):
+    pb = PROBLEMS[name]
+    print(f"[matvec fp64] {name} n_elems={pb.n_elems}", flush=True)
+    ctx.set_problem(pb)
+    x = np.random.default_rng(0).standard_normal((pb.n_nodes, pb.dim))
+    y = ctx.matvec(x, F.FP64)
+    yo = O.matvec(pb, x, np.float64)
+    assert rel(y, yo) < 1e-12      # FP64, different summation order only
+
+
+@pytest.mark.parametrize("name", ["plate", "cube", "cube_scaled"])
+def test_matvec_fp32_vs_oracle(ctx, name):
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    x = np.random.default_rng(1).standard_normal((pb.n_nodes, pb.dim))
+    y = ctx.matvec(x, F.FP32)
+    yo = O.matvec(pb, x, np.float64)
+    assert rel(y, yo) < 2e-6       # FP32 arithmetic
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_diag_and_rhs_vs_oracle(ctx, name):
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    assert rel(ctx.diag(), O.diag(pb, np.float64)) < 1e-13
+    if pb.convention == F.LOADS_REFERENCE:
+        assert rel(ctx.rhs(), O.rhs(pb, np.float64)) < 1e-14
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_static_fp64_vs_oracle_same_rule(ctx, name):
+    """FP64 PCG with the reference's stopping rule (gamma < 1e-8 gamma0)
+    against the FP64 restatement of gpuPCG_EbE_vec: rel L2 <= 1e-6."""
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    u, st = ctx.solve(F.FP64, tol=1e-8, tol_kind=F.TOL_REF_GAMMA)
+    s, uo, it = O.static_solve(pb, np.float64, eps=1e-8)
+    assert s == 0
+    assert abs(st["iterations"] - it) <= 1
+    assert rel(u, uo) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["rect_pcg", "9task_3"])
+def test_static_vs_golden_fixture(ctx, name):
+    """Golden fixtures: per-component L2 difference < 1e-5 (run_tests.py:42,67-69)."""
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    u, st = ctx.solve(F.FP64, tol=1e-8, tol_kind=F.TOL_REF_GAMMA)
+    ug = fixture_displacements("test_rect_pcg" if name == "rect_pcg" else name)
+    for c in range(2):
+        assert np.linalg.norm(u[:, c] - ug[:, c]) < 1e-5
+    # stresses sigma = D B u_e: the fixtures print 6 digits
+    s = ctx.stress(u)
+    sg = fixture_stress("test_rect_pcg" if name == "rect_pcg" else name)
+    assert np.abs(s - sg).max() / np.abs(sg).max() < 1e-4
+
+
+@pytest.mark.parametrize("name", ["plate", "cube", "cube_scaled"])
+def test_static_converged_vs_dense(ctx, name):
+    """Tight FP64 solve equals the dense direct solve of the oracle's operator."""
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    u, st = ctx.solve(F.FP64, tol=1e-13, tol_kind=F.TOL_REL_RES)
+    from helpers import dense_K
+    K = dense_K(pb)
+    f = O.rhs(pb, np.float64).ravel()
+    uo = np.linalg.solve(K, f)
+    assert rel(u, uo) < 1e-10
+    assert st["rel_res"] < 1e-12
+
+
+@pytest.mark.parametrize("name", ["plate", "cube", "cube_scaled"])
+def test_mixed_vs_fp64(ctx, name):
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    u64, s64 = ctx.solve(F.FP64, tol=1e-10, tol_kind=F.TOL_REL_RES)
+    um, sm = ctx.solve(F.MIXED, tol=1e-10, tol_kind=F.TOL_REL_RES)
+    assert sm["rel_res"] <= 1e-10 * 1.0001
+    assert rel(um, u64) < 1e-7
+    u32, s32 = ctx.solve(F.FP32, tol=1e-5, tol_kind=F.TOL_REL_RES)
+    assert rel(u32, u64) < 1e-3
+
+
+def test_deterministic(ctx):
+    pb = PROBLEMS["cube"]
+    ctx.set_problem(pb)
+    a, _ = ctx.solve(F.FP64, tol=1e-8)
+    b, _ = ctx.solve(F.FP64, tol=1e-8)
+    assert np.array_equal(a, b)
+    x = np.random.default_rng(7).standard_normal((pb.n_nodes, pb.dim))
+    assert np.array_equal(ctx.matvec(x), ctx.matvec(x))
+
+
+def test_stress_vs_oracle(ctx):
+    pb = PROBLEMS["cube_scaled"]
+    ctx.set_problem(pb)
+    u = np.random.default_rng(2).standard_normal((pb.n_nodes, pb.dim))
+    assert rel(ctx.stress(u), O.stress(pb, u, np.float64)) < 1e-12
+
+
+def test_zero_rhs_gives_zero(ctx):
+    pb = plate_problem(6, 3)
+    pb.face_p = np.zeros_like(pb.face_p)
+    ctx.set_problem(pb)
+    u, st = ctx.solve(F.FP64, tol=1e-8)
+    assert np.all(u == 0) and st["iterations"] == 0
