@@ -61,15 +61,34 @@ def test_diag_and_rhs_vs_oracle(ctx, name):
 
 @pytest.mark.parametrize("name", list(PROBLEMS))
 def test_static_fp64_vs_oracle_same_rule(ctx, name):
-    """FP64 PCG with the reference's stopping rule (gamma < 1e-8 gamma0)
-    against the FP64 restatement of gpuPCG_EbE_vec: rel L2 <= 1e-6."""
+    """FP64 PCG with the reference's stopping rule (gamma < 1e-8 gamma0,
+    femfunc.cpp:20,92) against the FP64 restatement of gpuPCG_EbE_vec.
+    At this loose rule the iterate is still ~1e-5 from the exact solution and
+    CG's path depends on summation order at that level (reordering FP64 dot
+    sums alone moves it by ~1.5e-6), so the bound is: the GPU result is as
+    close to the oracle as the oracle is to the converged solution, and the
+    iteration counts agree within one."""
     pb = PROBLEMS[name]
     ctx.set_problem(pb)
     u, st = ctx.solve(F.FP64, tol=1e-8, tol_kind=F.TOL_REF_GAMMA)
     s, uo, it = O.static_solve(pb, np.float64, eps=1e-8)
     assert s == 0
     assert abs(st["iterations"] - it) <= 1
+    s, uc, _ = O.static_solve(pb, np.float64, eps=1e-26)
+    assert rel(u, uo) <= max(1e-10, 1.0 * rel(uo, uc))
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_static_fp64_vs_oracle_converged(ctx, name):
+    """FP64 mode, reference rule tightened to gamma < 1e-24 gamma0 on both
+    sides: relative L2 difference <= 1e-6 (observed ~1e-12)."""
+    pb = PROBLEMS[name]
+    ctx.set_problem(pb)
+    u, st = ctx.solve(F.FP64, tol=1e-24, tol_kind=F.TOL_REF_GAMMA)
+    s, uo, it = O.static_solve(pb, np.float64, eps=1e-24)
+    assert s == 0
     assert rel(u, uo) < 1e-6
+    assert rel(ctx.stress(u), O.stress(pb, uo, np.float64)) < 1e-6
 
 
 @pytest.mark.parametrize("name", ["rect_pcg", "9task_3"])
