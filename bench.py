@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the EbE linear-elasticity solve path on one MI355X.
+
+Workload (N=1): 3D static tetrahedra on the unit cube, 64x64x64 hexes split
+into 6 Kuhn tets each (1,572,864 elements, 274,625 nodes, 823,875 DOF),
+z=0 face clamped, uniform pressure 1e3 on the z=1 face, E=2e7, nu=0.25.
+One "step" is one complete static solve to ||f - K u||_2 / ||f||_2 <= 1e-8
+with mixed precision (FP32 EbE matvec + FP32 Jacobi-PCG inside FP64
+iterative refinement), mesh and right-hand side already resident in HBM.
+
+Printed (one JSON line on stdout): time-to-solve of the mixed solve (the
+headline value, seconds, lower is better) plus the FP64 PCG time-to-solve,
+the mixed/FP64 speedup and displacement difference, PCG iterations/s, EbE
+matvec DOF-matvecs/s and GB/s in FP32 and FP64, the roofline of the dominant
+kernel and the CPU baseline (the oracle's FP32 restatement of the reference's
+single-threaded CPU PCG, timed on a bounded sample and extrapolated).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_matvec(ne, nn, ndof, s):
+    """Algorithmic bytes of one EbE matvec y = K x: int32 connectivity
+    (4 per node of each element), node coordinates, read x, write y; each
+    array counted once."""
+    return 4 * 4 * ne + s * 3 * nn + 2 * s * ndof
+
+
+def read_traffic(path):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for ln in f:
+            if ":" in ln:
+                k, v = ln.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    if "fetch_bytes_per_launch" in vals and "write_bytes_per_launch" in vals:
+        return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
+    return None
+
+
+def cpu_baseline(pb, n_fp64_iters, cpu_iters):
+    """The oracle's FP32 restatement of the reference's CPU path (element-local
+    vectors, per-DOF sums in element order, single thread): setup + a bounded
+    number of PCG iterations, extrapolated to the iteration count of the
+    converged solve."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    setup_s, iter_s = O.static_time_f32(pb, cpu_iters)
+    per_it = iter_s / cpu_iters
+    est = setup_s + per_it * n_fp64_iters
+    return {
+        "value": est, "unit": "s", "cores": 1, "kind": "port",
+        "sample": (f"cube64 (1,572,864 tets): setup {setup_s:.2f} s + {cpu_iters} PCG iterations "
+                   f"{iter_s:.2f} s on 1 core, extrapolated to {n_fp64_iters} iterations"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64, help="hexes per cube edge")
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--matvec-reps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import cudafemproject_amd as F
+    from cudafemproject_amd.problem import cube_problem
+
+    t0 = time.perf_counter()
+    pb = cube_problem(args.n, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)
+    ctx = F.Context(0)
+    ctx.set_problem(pb)
+    ctx.solve(F.FP64, tol=1e-3, tol_kind=F.TOL_REL_RES)  # builds diag, RHS, buffers
+    log(f"setup {time.perf_counter() - t0:.2f} s: {pb.n_elems} elems, {pb.n_nodes} nodes, "
+        f"{pb.n_dof} dof")
+
+    def run(prec, k):
+        out = []
+        for _ in range(k):
+            u, st = ctx.solve(prec, tol=1e-8, tol_kind=F.TOL_REL_RES)
+            out.append((u, st))
+        return out
+
+    for _ in range(args.warmup):
+        run(F.FP64, 1)
+        run(F.MIXED, 1)
+    ctx.sync()
+    r64 = run(F.FP64, args.steps)
+    ctx.sync()
+    t_wall = time.perf_counter()
+    rmx = run(F.MIXED, args.steps)
+    ctx.sync()
+    t_wall = time.perf_counter() - t_wall
+    t64 = np.mean([s["solve_ms"] for _, s in r64]) / 1e3
+    tmx = np.mean([s["solve_ms"] for _, s in rmx]) / 1e3
+    u64, s64 = r64[-1]
+    umx, smx = rmx[-1]
+    err = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+    log(f"fp64: {t64 * 1e3:.2f} ms, {s64['iterations']} it, rel_res {s64['rel_res']:.2e}")
+    log(f"mixed: {tmx * 1e3:.2f} ms, {smx['iterations']} it ({smx['outer_iterations']} outer), "
+        f"rel_res {smx['rel_res']:.2e}, |u_mixed-u_fp64|/|u_fp64| {err:.2e}")
+
+    mv32 = ctx.time_matvec(F.FP32, args.matvec_reps) / 1e3
+    mv64 = ctx.time_matvec(F.FP64, args.matvec_reps) / 1e3
+    b32 = bytes_per_matvec(pb.n_elems, pb.n_nodes, pb.n_dof, 4)
+    b64 = bytes_per_matvec(pb.n_elems, pb.n_nodes, pb.n_dof, 8)
+    log(f"matvec fp32 {mv32 * 1e6:.2f} us ({b32 / mv32 / 1e9:.0f} GB/s), "
+        f"fp64 {mv64 * 1e6:.2f} us ({b64 / mv64 / 1e9:.0f} GB/s)")
+
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "ebe_matvec_fp32_cube64_traffic.txt"))
+    achieved = b32 / mv32 / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "EbE matvec fp32 (k_ebe_patch + k_iface_gather), cube64",
+                "bytes_per_launch": b32}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(pb, s64["iterations"], args.cpu_iters)
+        log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['sample']})")
+
+    line = {
+        "metric": "3D PCG time-to-solve @ rel-res 1e-8, mixed vs FP64; EbE matvec HBM GB/s",
+        "value": float(tmx), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(tmx * 1e3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"cube {args.n}^3 hexes x 6 Kuhn tets, z=0 clamped, p=1e3 on z=1, "
+                               "static PCG to rel-res 1e-8",
+                   "n_elems": pb.n_elems, "n_nodes": pb.n_nodes, "n_dof": pb.n_dof,
+                   "tol": 1e-8, "solver": "mixed (fp32 pcg in fp64 refinement)"},
+        "fp64_time_to_solve_s": float(t64),
+        "speedup_mixed_vs_fp64": float(t64 / tmx),
+        "mixed_vs_fp64_rel_displacement_diff": err,
+        "iterations_fp64": s64["iterations"], "iterations_mixed": smx["iterations"],
+        "outer_iterations_mixed": smx["outer_iterations"],
+        "rel_res_fp64": s64["rel_res"], "rel_res_mixed": smx["rel_res"],
+        "pcg_it_per_s_fp64": s64["iterations"] / t64, "pcg_it_per_s_mixed": smx["iterations"] / tmx,
+        "wall_s_per_mixed_solve_incl_u_to_host": t_wall / args.steps,
+        "matvec_us_fp32": mv32 * 1e6, "matvec_us_fp64": mv64 * 1e6,
+        "matvec_gbs_fp32": b32 / mv32 / 1e9, "matvec_gbs_fp64": b64 / mv64 / 1e9,
+        "dof_matvecs_per_s_fp32": pb.n_dof / mv32, "dof_matvecs_per_s_fp64": pb.n_dof / mv64,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,7 @@ SIGNATURES = {
     "femgpu_set_stream": (C.c_int, [_P, _P]),
     "femgpu_sync": (C.c_int, [_P]),
     "femgpu_set_mesh": (C.c_int, [_P, C.c_int, C.c_int64, _D, C.c_int64, _I]),
+    "femgpu_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
     "femgpu_set_material": (C.c_int, [_P, C.c_double, C.c_double, _D]),
     "femgpu_set_constraints": (C.c_int, [_P, C.c_int64, _I, _I]),
     "femgpu_set_pressure_faces": (C.c_int, [_P, C.c_int64, _I, _I, _D, _D]),
@@ -76,6 +77,7 @@ SIGNATURES = {
     "femgpu_diag": (C.c_int, [_P, _D]),
     "femgpu_rhs": (C.c_int, [_P, _D]),
     "femgpu_matvec_repeat": (C.c_int, [_P, C.c_int, C.c_int]),
+    "femgpu_time_matvec": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "femgpu_default_solve_opts": (C.c_int, [C.POINTER(SolveOpts)]),
     "femgpu_solve_static": (C.c_int, [_P, C.POINTER(SolveOpts), _D, C.POINTER(Stats)]),
     "femgpu_stress": (C.c_int, [_P, _D, _D]),
@@ -166,6 +168,9 @@ class Context:
                                              elems.shape[0], _ip(elems)), "set_mesh")
         self.dim, self.n_nodes, self.n_elems = dim, nodes.shape[0], elems.shape[0]
 
+    def set_tuning(self, elems_per_patch: int = 0, patch_threads: int = 0):
+        self._check(self.lib.femgpu_set_tuning(self.h, elems_per_patch, patch_threads), "set_tuning")
+
     def set_material(self, E: float, nu: float, elem_scale: np.ndarray | None = None):
         if elem_scale is not None:
             elem_scale = np.ascontiguousarray(elem_scale, dtype=np.float64)
@@ -240,6 +245,11 @@ class Context:
     def matvec_repeat(self, precision: int, reps: int):
         self._check(self.lib.femgpu_matvec_repeat(self.h, precision, reps), "matvec_repeat")
 
+    def time_matvec(self, precision: int, reps: int) -> float:
+        ms = C.c_double(0)
+        self._check(self.lib.femgpu_time_matvec(self.h, precision, reps, C.byref(ms)), "time_matvec")
+        return ms.value
+
     def set_stream(self, stream_ptr: int | None):
         self._check(self.lib.femgpu_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None),
                     "set_stream")
@@ -249,7 +259,7 @@ class Context:
 
     # --- solves ----------------------------------------------------------------
     def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
-              max_iter: int = 0, inner_tol: float = 1e-4, check_every: int = 32,
+              max_iter: int = 0, inner_tol: float = 1e-3, check_every: int = 32,
               raise_on_error: bool = True):
         o = SolveOpts()
         self.lib.femgpu_default_solve_opts(C.byref(o))

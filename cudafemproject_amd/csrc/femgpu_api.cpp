@@ -84,9 +84,11 @@ struct femgpu_ctx {
   std::vector<uint8_t> fix_user;  // per user node
   std::vector<int32_t> ctype;     // per user node (last line wins)
 
-  bool dirty_mat = true, dirty_fix = true, dirty_diag = true;
+  bool dirty_mat = true, dirty_fix = true, dirty_diag = true, dirty_rhs = true;
 
-  CgScalars *h_sc = nullptr;  // pinned
+  CgScalars *h_sc = nullptr;  // pinned, h_sc[0..3]: poll ring
+  hipEvent_t ev_poll[4] = {nullptr, nullptr, nullptr, nullptr};
+  int emax_req = 0, threads_req = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   // graph cache for PCG iteration blocks, keyed by everything baked into
@@ -400,18 +402,33 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
     (void)hipGraphDestroy(g);
     ctx->graphs[key] = exec;
   }
+  // Replays are pipelined: replay k+1 is queued before the host looks at the
+  // scalars copied after replay k (kernels of a finished solve exit at once).
   // Every replay either advances the device iteration counter or sets the
   // stop flag; anything else means the kernels did not run.
-  int last_iter = -1;
-  const long max_replays = long(max_iter) / block + 2;
-  for (long rep = 0;; ++rep) {
+  const long max_replays = long(max_iter) / block + 3;
+  auto launch = [&](int slot) -> int {
     HIPCHK(hipGraphLaunch(exec, ctx->stream));
-    if ((st = poll(ctx, sc))) return st;
-    if (ctx->h_sc->done) break;
-    if (ctx->h_sc->iter <= last_iter || rep > max_replays)
+    HIPCHK(hipMemcpyAsync(&ctx->h_sc[slot], sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev_poll[slot], ctx->stream));
+    return FEMGPU_OK;
+  };
+  if ((st = launch(0))) return st;
+  int cur = 0, last_iter = -1;
+  for (long rep = 0;; ++rep) {
+    const int nxt = (cur + 1) & 1;
+    if ((st = launch(nxt))) return st;
+    HIPCHK(hipEventSynchronize(ctx->ev_poll[cur]));
+    const CgScalars &h = ctx->h_sc[cur];
+    if (h.done) break;
+    if (h.iter <= last_iter || rep > max_replays)
       return fail(ctx, FEMGPU_E_HIP, "PCG made no progress (kernels not running)");
-    last_iter = ctx->h_sc->iter;
+    last_iter = h.iter;
+    cur = nxt;
   }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->h_sc[0] = ctx->h_sc[(cur + 1) & 1];
   *iters = ctx->h_sc->iter;
   *done_code = ctx->h_sc->done;
   return FEMGPU_OK;
@@ -477,7 +494,8 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   HIPCHK(hipSetDevice(device_ordinal));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   ctx->own_stream = true;
-  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_sc), sizeof(CgScalars), 0));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_sc), 4 * sizeof(CgScalars), 0));
+  for (auto &e : ctx->ev_poll) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
   *out = c.release();
@@ -494,6 +512,8 @@ void femgpu_destroy(femgpu_ctx *ctx) {
   if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (auto &e : ctx->ev_poll)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -585,6 +605,7 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   dm.n_owned = P.n_owned;
   dm.n_iface = P.n_iface;
   dm.n_active = P.n_owned + P.n_iface;
+  dm.patch_threads = ctx->threads_req ? ctx->threads_req : 512;
   int st;
   femgpu::PatchInfo *pp;
   uint32_t *lid;
@@ -619,8 +640,21 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   dm.coords32 = pc32;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->have_mesh = true;
-  ctx->dirty_mat = ctx->dirty_fix = ctx->dirty_diag = true;
+  ctx->dirty_mat = ctx->dirty_fix = ctx->dirty_diag = ctx->dirty_rhs = true;
   return ensure_fix(ctx);
+}
+
+int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads) {
+  if (!ctx) return FEMGPU_E_ARG;
+  if (elems_per_patch != 0 && (elems_per_patch < 16 || elems_per_patch > femgpu::EMAX_LIMIT))
+    return fail(ctx, FEMGPU_E_ARG, "elems_per_patch out of range");
+  if (patch_threads != 0 && patch_threads != 256 && patch_threads != 512)
+    return fail(ctx, FEMGPU_E_ARG, "patch_threads must be 256 or 512");
+  ctx->emax = elems_per_patch;
+  ctx->threads_req = patch_threads;
+  if (patch_threads) ctx->dm.patch_threads = patch_threads;
+  drop_graphs(ctx);
+  return FEMGPU_OK;
 }
 
 int femgpu_set_material(femgpu_ctx *ctx, double E, double nu, const double *elem_scale) {
@@ -661,6 +695,7 @@ int femgpu_set_constraints(femgpu_ctx *ctx, int64_t n, const int32_t *node, cons
   ctx->ccode.assign(code, code + n);
   ctx->dirty_fix = true;
   ctx->dirty_diag = true;
+  ctx->dirty_rhs = true;
   return FEMGPU_OK;
 }
 
@@ -687,6 +722,7 @@ int femgpu_set_pressure_faces(femgpu_ctx *ctx, int64_t n, const int32_t *face_no
     if (!(l2 > 0)) return fail(ctx, FEMGPU_E_BAD_MESH, "zero face normal");
     for (int j = 0; j < d; ++j) ctx->fnormal[f * d + j] = normal[f * d + j] / std::sqrt(l2);
   }
+  ctx->dirty_rhs = true;
   return FEMGPU_OK;
 }
 
@@ -698,6 +734,7 @@ int femgpu_set_nodal_loads(femgpu_ctx *ctx, int64_t n, const int32_t *node, cons
     if (node[i] < 0 || node[i] >= ctx->nn) return fail(ctx, FEMGPU_E_ARG, "load node out of range");
   ctx->lnode.assign(node, node + n);
   ctx->lvec.assign(vec, vec + n * ctx->dim);
+  ctx->dirty_rhs = true;
   return FEMGPU_OK;
 }
 
@@ -706,6 +743,7 @@ int femgpu_set_load_convention(femgpu_ctx *ctx, int convention) {
   if (convention != FEMGPU_LOADS_REFERENCE && convention != FEMGPU_LOADS_PHYSICAL)
     return fail(ctx, FEMGPU_E_ARG, "unknown load convention");
   ctx->conv = convention;
+  ctx->dirty_rhs = true;
   return FEMGPU_OK;
 }
 
@@ -716,6 +754,7 @@ int femgpu_set_extra_rhs(femgpu_ctx *ctx, const double *f) {
     ctx->extra_rhs.assign(f, f + size_t(ctx->nn) * ctx->dim);
   else
     ctx->extra_rhs.clear();
+  ctx->dirty_rhs = true;
   return FEMGPU_OK;
 }
 
@@ -774,6 +813,21 @@ int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps) {
   return FEMGPU_OK;
 }
 
+int femgpu_time_matvec(femgpu_ctx *ctx, int precision, int reps, double *avg_ms) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (reps < 1 || !avg_ms) return FEMGPU_E_ARG;
+  if ((st = femgpu_matvec_repeat(ctx, precision, 1))) return st;  // buffers + warm-up
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  if ((st = femgpu_matvec_repeat(ctx, precision, reps))) return st;
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *avg_ms = double(ms) / reps;
+  return FEMGPU_OK;
+}
+
 int femgpu_diag(femgpu_ctx *ctx, double *d) {
   int st = need_mesh(ctx);
   if (st) return st;
@@ -805,7 +859,7 @@ int femgpu_default_solve_opts(femgpu_solve_opts *o) {
   o->tol_kind = FEMGPU_TOL_REF_GAMMA;
   o->tol = 1e-8;
   o->max_iter = 0;
-  o->inner_tol = 1e-4;
+  o->inner_tol = 1e-3;
   o->check_every = 32;
   return FEMGPU_OK;
 }
@@ -825,12 +879,15 @@ int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *
   const int block = o.check_every > 0 ? o.check_every : 32;
   if ((st = ensure_diag(ctx))) return st;
   const size_t nd = size_t(ctx->nn) * ctx->dim;
-  std::vector<double> fu, fi;
-  host_rhs(ctx, false, fu);
-  fi = to_internal(ctx, fu.data());
   double *f64, *x64;
   CgScalars *sc, *sc2;
-  if ((st = upload(ctx, "f64", fi, &f64)) || (st = dbuf(ctx, "sol64", nd, &x64)) ||
+  if (ctx->dirty_rhs) {
+    std::vector<double> fu;
+    host_rhs(ctx, false, fu);
+    if ((st = upload(ctx, "f64", to_internal(ctx, fu.data()), &f64))) return st;
+    ctx->dirty_rhs = false;
+  }
+  if ((st = dbuf(ctx, "f64", nd, &f64)) || (st = dbuf(ctx, "sol64", nd, &x64)) ||
       (st = dbuf(ctx, "sc_main", 1, &sc)) || (st = dbuf(ctx, "sc_outer", 1, &sc2)))
     return st;
   femgpu_stats S{};
@@ -883,9 +940,12 @@ int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *
     double rel = fnorm > 0 ? 1.0 : 0.0;
     int outer = 0;
     int status = FEMGPU_OK;
-    const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+    const double floor_tol = o.inner_tol > 0 ? o.inner_tol : 1e-3;
     while (rel > o.tol && outer < 50) {
       femgpu::launch_to_f32_scaled(ctx->dm, r64, w.r, w.x, sc2, ctx->stream);
+      // ask the FP32 solve only for the reduction still missing (with a
+      // factor 2 of margin), but never below what FP32 can deliver
+      const double inner = std::min(0.5, std::max(floor_tol, 0.5 * o.tol / rel));
       int it = 0;
       if ((st = run_pcg<float>(ctx, w, OpShift<float>(), FEMGPU_TOL_REL_RES, inner,
                                std::max(1, max_iter - iters), block, 1, sc, &it, &done)))

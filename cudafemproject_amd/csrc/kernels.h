@@ -15,6 +15,7 @@ struct DevMesh {
   int np = 0;              // patches
   int emax = 0;            // LDS stride (elements per patch bound)
   int pn = 0;              // LDS node capacity
+  int patch_threads = 256; // threads per patch workgroup (256 or 512)
   int64_t nn = 0, ne = 0, n_owned = 0, n_iface = 0, n_active = 0;
   const PatchInfo *patches = nullptr;
   const uint32_t *elem_lid = nullptr;
@@ -47,8 +48,7 @@ struct CgScalars {
   int done;            // 0 running, 1 converged, 2 non-finite, 3 breakdown, 4 max-iter
   int iter;
   int max_iter;
-  unsigned cnt_a;      // last-block tickets
-  unsigned cnt_b;
+  int pad0, pad1;
   double scale;        // scratch scalar (norms for refinement)
 };
 

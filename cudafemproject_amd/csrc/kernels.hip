@@ -36,7 +36,7 @@ constexpr int NT = 256;          // threads per block for every kernel
 constexpr int NODE_GRID_MAX = 2048;
 
 struct LdsLayout {
-  int off_c, off_x, off_res, off_end, off_inc, off_fix, off_red, off_flag, total;
+  int off_c, off_x, off_res, off_end, off_inc, off_fix, off_red, total;
 };
 
 __host__ __device__ inline int al16(int v) { return (v + 15) & ~15; }
@@ -51,8 +51,7 @@ __host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ema
   L.off_inc = L.off_end + al16(pn * 2);
   L.off_fix = L.off_inc + al16(emax * 4 * 2);
   L.off_red = L.off_fix + al16(pn);
-  L.off_flag = L.off_red + al16((NT / 64) * 8);
-  L.total = L.off_flag + 16;
+  L.total = L.off_red + 16 * 8;
   return L;
 }
 
@@ -63,6 +62,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block-wide sum, same value returned in every thread; fixed order.
+template <int BT = NT>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -71,30 +71,8 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   __syncthreads();
   double r = 0;
 #pragma unroll
-  for (int i = 0; i < NT / 64; ++i) r += sh[i];
+  for (int i = 0; i < BT / 64; ++i) r += sh[i];
   return r;
-}
-
-__device__ __forceinline__ void store_agent(double *p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double *p) {
-  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Thread 0 of each block publishes its partial first; returns true in all
-// threads of the block that arrives last (it then owns the final reduction).
-__device__ __forceinline__ bool arrive_last(unsigned *counter, unsigned total, int *s_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    *s_flag = (t == total - 1);
-  }
-  __syncthreads();
-  const bool last = *s_flag != 0;
-  if (last) __threadfence();
-  return last;
 }
 
 // ---------------------------------------------------------------------------
@@ -301,8 +279,8 @@ struct PatchArgs {
   double *part;
 };
 
-template <typename T, int DIM, int MODE>
-__global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
+template <typename T, int DIM, int MODE, int PT>
+__global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   constexpr int NPE = DIM + 1, NDE = DIM * NPE;
   extern __shared__ __align__(16) unsigned char lds[];
   const int b = blockIdx.x;
@@ -319,7 +297,6 @@ __global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
   uint16_t *s_inc = reinterpret_cast<uint16_t *>(lds + L.off_inc);
   uint8_t *s_fix = reinterpret_cast<uint8_t *>(lds + L.off_fix);
   double *s_red = reinterpret_cast<double *>(lds + L.off_red);
-  int *s_flag = reinterpret_cast<int *>(lds + L.off_flag);
 
   const PatchInfo P = a.patches[pid];
   const int nloc = P.n_own + P.n_sh;
@@ -329,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
   const T cM = MODE == 1 ? a.sh.cM : T(0);
 
   // Phase A: stage the patch's nodes in LDS.
-  for (int k = threadIdx.x; k < nloc; k += NT) {
+  for (int k = threadIdx.x; k < nloc; k += PT) {
     const bool own = k < P.n_own;
     const int64_t gid = own ? int64_t(P.own_start + k) : int64_t(a.sh_gid[P.sh_start + k - P.n_own]);
 #pragma unroll
@@ -362,12 +339,12 @@ __global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
     s_fix[k] = a.fixmask[gid];
     s_end[k] = a.node_end[P.pnode_start + k];
   }
-  for (int j = threadIdx.x; j < P.n_elem * NPE; j += NT)
+  for (int j = threadIdx.x; j < P.n_elem * NPE; j += PT)
     s_inc[j] = a.inc[int64_t(P.elem_start) * NPE + j];
   __syncthreads();
 
   // Phase B: one element per thread, forces into LDS.
-  for (int i = threadIdx.x; i < P.n_elem; i += NT) {
+  for (int i = threadIdx.x; i < P.n_elem; i += PT) {
     const uint32_t lid = a.elem_lid[P.elem_start + i];
     T X[NDE], U[NDE];
     unsigned fix = 0;
@@ -395,7 +372,7 @@ __global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
   __syncthreads();
 
   // Phase C: fixed-order node sums; owned nodes final, interface to slots.
-  for (int k = threadIdx.x; k < nloc; k += NT) {
+  for (int k = threadIdx.x; k < nloc; k += PT) {
     const int b0 = k ? int(s_end[k - 1]) : 0, b1 = int(s_end[k]);
     T acc[DIM];
 #pragma unroll
@@ -425,25 +402,8 @@ __global__ __launch_bounds__(NT) void k_ebe_patch(PatchArgs<T> a) {
   }
 
   if constexpr (MODE == 1) {
-    const double tot = block_sum(acc_en, s_red);
-    if (threadIdx.x == 0) store_agent(&a.part[pid], tot);
-    if (arrive_last(&a.sc->cnt_a, unsigned(a.np), s_flag)) {
-      double v = 0.0;
-      for (int i = threadIdx.x; i < a.np; i += NT) v += load_agent(&a.part[i]);
-      v = block_sum(v, s_red);
-      if (threadIdx.x == 0) {
-        CgScalars *sc = a.sc;
-        sc->pq = v;
-        const double alpha = sc->gamma / v;
-        if (!(v > 0.0) || !isfinite(alpha)) {
-          sc->alpha = 0.0;
-          sc->done = isfinite(v) ? 3 : 2;
-        } else {
-          sc->alpha = alpha;
-        }
-        __hip_atomic_store(&sc->cnt_a, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    const double tot = block_sum<PT>(acc_en, s_red);
+    if (threadIdx.x == 0) a.part[pid] = tot;
   }
 }
 
@@ -483,7 +443,6 @@ struct UpdArgs {
 template <typename T, int DIM>
 __global__ __launch_bounds__(NT) void k_pcg_update(UpdArgs<T> a) {
   __shared__ double s_red[NT / 64];
-  __shared__ int s_flag;
   if (a.sc->done) return;
   const T alpha = T(a.sc->alpha), beta = T(a.sc->beta);
   double rz = 0.0, rr = 0.0;
@@ -525,33 +484,93 @@ __global__ __launch_bounds__(NT) void k_pcg_update(UpdArgs<T> a) {
   rz = block_sum(rz, s_red);
   rr = block_sum(rr, s_red);
   if (threadIdx.x == 0) {
-    store_agent(&a.part_a[blockIdx.x], rz);
-    store_agent(&a.part_b[blockIdx.x], rr);
+    a.part_a[blockIdx.x] = rz;
+    a.part_b[blockIdx.x] = rr;
   }
-  if (arrive_last(&a.sc->cnt_b, gridDim.x, &s_flag)) {
-    double g = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < int(gridDim.x); i += NT) {
-      g += load_agent(&a.part_a[i]);
-      q += load_agent(&a.part_b[i]);
+}
+
+// Finalize of the p'q reduction (one block): alpha = gamma / p'q.
+__global__ __launch_bounds__(NT) void k_fin_alpha(CgScalars *sc, const double *part, int n) {
+  __shared__ double s_red[NT / 64];
+  if (sc->done) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) v += part[i];
+  v = block_sum(v, s_red);
+  if (threadIdx.x == 0) {
+    sc->pq = v;
+    const double alpha = sc->gamma / v;
+    if (!(v > 0.0) || !isfinite(alpha)) {
+      sc->alpha = 0.0;
+      sc->done = isfinite(v) ? 3 : 2;
+    } else {
+      sc->alpha = alpha;
     }
-    g = block_sum(g, s_red);
-    q = block_sum(q, s_red);
-    if (threadIdx.x == 0) {
-      CgScalars *sc = a.sc;
-      sc->iter += 1;
-      sc->rr = q;
-      sc->scale = g;  // last gamma_new
-      const bool stop = sc->kind == 0 ? (g < sc->thresh) : (q <= sc->thresh);
-      if (stop) {
-        sc->done = 1;
-      } else {
-        sc->beta = g / sc->gamma;
-        sc->gamma = g;
-        if (!isfinite(g) || !isfinite(sc->beta)) sc->done = 2;
-      }
-      if (!sc->done && sc->iter >= sc->max_iter) sc->done = 4;
-      __hip_atomic_store(&sc->cnt_b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Finalize of the r'z, r'r reductions (one block): stop test, beta
+// (femfunc.cpp:81-110 order: test gamma_new first, then beta, then NaN check).
+__global__ __launch_bounds__(NT) void k_fin_beta(CgScalars *sc, const double *pa, const double *pb,
+                                                 int n) {
+  __shared__ double s_red[NT / 64];
+  if (sc->done) return;
+  double g = 0.0, q = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    g += pa[i];
+    q += pb[i];
+  }
+  g = block_sum(g, s_red);
+  q = block_sum(q, s_red);
+  if (threadIdx.x == 0) {
+    sc->iter += 1;
+    sc->rr = q;
+    sc->scale = g;  // last gamma_new
+    const bool stop = sc->kind == 0 ? (g < sc->thresh) : (q <= sc->thresh);
+    if (stop) {
+      sc->done = 1;
+    } else {
+      sc->beta = g / sc->gamma;
+      sc->gamma = g;
+      if (!isfinite(g) || !isfinite(sc->beta)) sc->done = 2;
     }
+    if (!sc->done && sc->iter >= sc->max_iter) sc->done = 4;
+  }
+}
+
+// Finalize of the PCG initialisation (one block).
+__global__ __launch_bounds__(NT) void k_fin_init(CgScalars *sc, const double *pa, const double *pb,
+                                                 int n) {
+  __shared__ double s_red[NT / 64];
+  double g = 0.0, q = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    g += pa[i];
+    q += pb[i];
+  }
+  g = block_sum(g, s_red);
+  q = block_sum(q, s_red);
+  if (threadIdx.x == 0) {
+    sc->gamma0 = g;
+    sc->gamma = g;
+    sc->scale = g;
+    sc->rr0 = q;
+    sc->rr = q;
+    sc->alpha = 0.0;
+    sc->beta = 0.0;
+    sc->iter = 0;
+    sc->thresh = sc->kind == 0 ? sc->eps * g : sc->eps * sc->eps * q;
+    sc->done = q == 0.0 ? 1 : (isfinite(g) ? 0 : 2);
+  }
+}
+
+// Finalize of the residual norm (one block): rr and its square root.
+__global__ __launch_bounds__(NT) void k_fin_norm(CgScalars *sc, const double *part, int n) {
+  __shared__ double s_red[NT / 64];
+  double q = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) q += part[i];
+  q = block_sum(q, s_red);
+  if (threadIdx.x == 0) {
+    sc->rr = q;
+    sc->scale = sqrt(q);
   }
 }
 
@@ -559,7 +578,6 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_pcg_init(int64_t n, const T *r, T *z, const T *invD,
                                                  CgScalars *sc, double *part_a, double *part_b) {
   __shared__ double s_red[NT / 64];
-  __shared__ int s_flag;
   double rz = 0.0, rr = 0.0;
   const int64_t stride = int64_t(gridDim.x) * NT;
   for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
@@ -572,30 +590,8 @@ __global__ __launch_bounds__(NT) void k_pcg_init(int64_t n, const T *r, T *z, co
   rz = block_sum(rz, s_red);
   rr = block_sum(rr, s_red);
   if (threadIdx.x == 0) {
-    store_agent(&part_a[blockIdx.x], rz);
-    store_agent(&part_b[blockIdx.x], rr);
-  }
-  if (arrive_last(&sc->cnt_b, gridDim.x, &s_flag)) {
-    double g = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < int(gridDim.x); i += NT) {
-      g += load_agent(&part_a[i]);
-      q += load_agent(&part_b[i]);
-    }
-    g = block_sum(g, s_red);
-    q = block_sum(q, s_red);
-    if (threadIdx.x == 0) {
-      sc->gamma0 = g;
-      sc->gamma = g;
-      sc->scale = g;
-      sc->rr0 = q;
-      sc->rr = q;
-      sc->alpha = 0.0;
-      sc->beta = 0.0;
-      sc->iter = 0;
-      sc->thresh = sc->kind == 0 ? sc->eps * g : sc->eps * sc->eps * q;
-      sc->done = q == 0.0 ? 1 : (isfinite(g) ? 0 : 2);
-      __hip_atomic_store(&sc->cnt_b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    part_a[blockIdx.x] = rz;
+    part_b[blockIdx.x] = rr;
   }
 }
 
@@ -608,8 +604,6 @@ __global__ void k_reset_scalars(CgScalars *sc, int kind, double eps, int max_ite
     sc->done = 0;
     sc->iter = 0;
     sc->max_iter = max_iter;
-    sc->cnt_a = 0u;
-    sc->cnt_b = 0u;
     sc->scale = 0.0;
   }
 }
@@ -621,7 +615,6 @@ __global__ __launch_bounds__(NT) void k_residual64(int64_t n_owned, int64_t n_ac
                                                    const double *y, const double *partial,
                                                    double *r, CgScalars *sc, double *part) {
   __shared__ double s_red[NT / 64];
-  __shared__ int s_flag;
   double rr = 0.0;
   const int64_t stride = int64_t(gridDim.x) * NT;
   for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n_active; i += stride) {
@@ -645,17 +638,7 @@ __global__ __launch_bounds__(NT) void k_residual64(int64_t n_owned, int64_t n_ac
     }
   }
   rr = block_sum(rr, s_red);
-  if (threadIdx.x == 0) store_agent(&part[blockIdx.x], rr);
-  if (arrive_last(&sc->cnt_b, gridDim.x, &s_flag)) {
-    double q = 0.0;
-    for (int i = threadIdx.x; i < int(gridDim.x); i += NT) q += load_agent(&part[i]);
-    q = block_sum(q, s_red);
-    if (threadIdx.x == 0) {
-      sc->rr = q;
-      sc->scale = sqrt(q);
-      __hip_atomic_store(&sc->cnt_b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
 __global__ __launch_bounds__(NT) void k_to_f32_scaled(int64_t n, const double *r64, float *r32,
@@ -907,17 +890,25 @@ PatchArgs<T> patch_args(const DevMesh &dm) {
   return a;
 }
 
-template <typename T, int DIM, int MODE>
-void launch_patch(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
+template <typename T, int DIM, int MODE, int PT>
+void launch_patch_pt(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
   const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.emax);
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ebe_patch<T, DIM, MODE>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ebe_patch<T, DIM, MODE, PT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
   const int grid = 8 * a.p8;
-  hipLaunchKernelGGL((k_ebe_patch<T, DIM, MODE>), dim3(grid), dim3(NT), L.total, s, a);
+  hipLaunchKernelGGL((k_ebe_patch<T, DIM, MODE, PT>), dim3(grid), dim3(PT), L.total, s, a);
+}
+
+template <typename T, int DIM, int MODE>
+void launch_patch(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
+  if (dm.patch_threads == 512)
+    launch_patch_pt<T, DIM, MODE, 512>(dm, a, s);
+  else
+    launch_patch_pt<T, DIM, MODE, 256>(dm, a, s);
 }
 
 }  // namespace
@@ -986,6 +977,7 @@ void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScala
   const int g = grid_for(n);
   hipLaunchKernelGGL((k_pcg_init<T>), dim3(g), dim3(NT), 0, s, n, r, z, invD, sc, part,
                      part + NODE_GRID_MAX);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, part, part + NODE_GRID_MAX, g);
 }
 
 template <typename T>
@@ -1004,6 +996,7 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial
     launch_patch<T, 3, 1>(dm, a, s);
   else
     launch_patch<T, 2, 1>(dm, a, s);
+  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, sc, part_a, dm.np);
   UpdArgs<T> u{};
   u.n_owned = dm.n_owned;
   u.n_active = dm.n_active;
@@ -1024,6 +1017,7 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial
     hipLaunchKernelGGL((k_pcg_update<T, 3>), dim3(g), dim3(NT), 0, s, u);
   else
     hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(g), dim3(NT), 0, s, u);
+  hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, sc, u.part_a, u.part_b, g);
 }
 
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
@@ -1035,6 +1029,7 @@ void launch_residual64(const DevMesh &dm, const double *f, const double *y, cons
   else
     hipLaunchKernelGGL((k_residual64<2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
                        dm.slot_start, f, y, partial, r, sc, part);
+  hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(NT), 0, s, sc, part, g);
 }
 
 void launch_to_f32_scaled(const DevMesh &dm, const double *r64, float *r32, float *x32,

@@ -102,6 +102,11 @@ int femgpu_sync(femgpu_ctx *ctx);
  * resets material, constraints and loads. */
 int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nodes,
                     int64_t n_elems, const int32_t *elems);
+/* Launch tuning: elements per patch (applies at the next femgpu_set_mesh;
+ * 0 = default 512) and threads per patch workgroup (256 or 512; 0 = keep).
+ * No counterpart in the reference (its kernels use 256 threads and one
+ * thread per element, fem_utils_kernels.cu:3). */
+int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads);
 /* Isotropic material (datakeeper.cpp:96-114, CreateMatrixD).  elem_scale:
  * optional per-element modulus factor (n_elems, user element order). */
 int femgpu_set_material(femgpu_ctx *ctx, double E, double nu, const double *elem_scale);
@@ -137,6 +142,9 @@ int femgpu_rhs(femgpu_ctx *ctx, double *f);
  * internal vectors, enqueued on the context stream without synchronising
  * (for timing with events on that stream). */
 int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps);
+/* Average device time (ms) of one standalone EbE matvec over `reps`
+ * back-to-back products, measured with HIP events on the context stream. */
+int femgpu_time_matvec(femgpu_ctx *ctx, int precision, int reps, double *avg_ms);
 
 /* --- static solve -----------------------------------------------------------
  * Replaces: gpuCalculateFEM_EbE_vec + gpuPCG_EbE_vec (femfunc.cpp:3-128). */
@@ -145,7 +153,8 @@ typedef struct {
   int tol_kind;       /* FEMGPU_TOL_REF_GAMMA / FEMGPU_TOL_REL_RES            */
   double tol;         /* 1e-8 in the reference's static path (femfunc.cpp:20) */
   int max_iter;       /* total PCG iteration cap; <= 0: 1,000,000             */
-  double inner_tol;   /* MIXED: relative residual reduction per FP32 solve    */
+  double inner_tol;   /* MIXED: smallest relative residual reduction asked of
+                         one FP32 inner solve (default 1e-3)                  */
   int check_every;    /* iterations between host polls of the stop flag       */
 } femgpu_solve_opts;
 
