@@ -259,7 +259,7 @@ class Context:
 
     # --- solves ----------------------------------------------------------------
     def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
-              max_iter: int = 0, inner_tol: float = 1e-3, check_every: int = 32,
+              max_iter: int = 0, inner_tol: float = 0.1, check_every: int = 32,
               raise_on_error: bool = True):
         o = SolveOpts()
         self.lib.femgpu_default_solve_opts(C.byref(o))

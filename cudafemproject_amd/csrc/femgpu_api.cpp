@@ -26,7 +26,7 @@ using femgpu::OpShift;
 
 namespace {
 
-constexpr int NODE_PART = 2 * 2048;  // per-block partials of node kernels (2 arrays)
+constexpr int NODE_PART = 4 * femgpu::NODE_PARTS;  // partials of node kernels (4 arrays)
 
 struct DevBuf {
   void *p = nullptr;
@@ -359,25 +359,11 @@ int pcg_ws(femgpu_ctx *ctx, const char *tag, PcgWs<T> &w) {
   return FEMGPU_OK;
 }
 
-// Runs PCG iterations until the device-side stop flag is set.  r holds the
-// initial residual, x must be zero.  Iteration blocks of `block` steps are
-// captured once into a HIP graph and replayed.
+// Graph of `block` fused PCG iterations (block even, so every replay starts
+// at parity 0) on these buffers; captured once and cached.
 template <typename T>
-int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, int max_iter,
-            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code) {
-  double *part_patch, *part_node;
-  int st;
-  if ((st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)) ||
-      (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
-    return st;
-  const int64_t nd = ctx->nn * ctx->dim;
-  femgpu::launch_fill<T>(w.x, nd, T(0), ctx->stream);
-  femgpu::launch_fill<T>(w.p, nd, T(0), ctx->stream);
-  femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
-  femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.z, w.invD, sc, part_node, ctx->stream);
-  HIPCHK(hipGetLastError());
-  if (block < 1) block = 1;
-  // graph for `block` iterations on these buffers
+int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_tag,
+              CgScalars *sc, double *part_patch, double *part_node, hipGraphExec_t *out) {
   if (ctx->graph_gen != ctx->buf_gen) {
     drop_graphs(ctx);
     ctx->graph_gen = ctx->buf_gen;
@@ -387,25 +373,35 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
                 (const void *)w.x, (const void *)w.invD, (const void *)sc, (const void *)sh.m,
                 (const void *)w.partial, (const void *)part_patch, double(sh.cK), double(sh.cM));
   const std::string key(keybuf);
-  hipGraphExec_t exec = nullptr;
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end()) {
-    exec = it->second;
-  } else {
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < block; ++k)
-      femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.z, w.p, w.q, w.partial, w.invD, sh, sc,
-                                 part_patch, part_node, ctx->stream);
-    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
-    HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-    ctx->graphs[key] = exec;
+    *out = it->second;
+    return FEMGPU_OK;
   }
-  // Replays are pipelined: replay k+1 is queued before the host looks at the
-  // scalars copied after replay k (kernels of a finished solve exit at once).
-  // Every replay either advances the device iteration counter or sets the
-  // stop flag; anything else means the kernels did not run.
+  hipGraph_t g;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < block; ++k)
+    femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.z, w.p, w.q, w.partial, w.invD, sh, sc,
+                               part_patch, part_node, k & 1, ctx->stream);
+  HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+  HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  ctx->graphs[key] = exec;
+  *out = exec;
+  return FEMGPU_OK;
+}
+
+// Replays the iteration graph until the device-side stop flag is set or,
+// when pause_rr > 0, until r'r has dropped below pause_rr.  Replays are
+// pipelined: replay k+1 is queued before the host looks at the scalars copied
+// after replay k (the kernels of a finished solve exit at once).  Every replay
+// either advances the iteration counter or sets the stop flag; anything else
+// means the kernels did not run.  On return the stream is idle and h_sc[0]
+// holds the last scalars.
+int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block, int max_iter,
+                 double pause_rr) {
+  int st;
   const long max_replays = long(max_iter) / block + 3;
   auto launch = [&](int slot) -> int {
     HIPCHK(hipGraphLaunch(exec, ctx->stream));
@@ -421,15 +417,39 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
     if ((st = launch(nxt))) return st;
     HIPCHK(hipEventSynchronize(ctx->ev_poll[cur]));
     const CgScalars &h = ctx->h_sc[cur];
-    if (h.done) break;
-    if (h.iter <= last_iter || rep > max_replays)
+    if (h.done || (pause_rr > 0 && h.rr_final < pause_rr)) break;
+    if (h.iter_final <= last_iter || rep > max_replays)
       return fail(ctx, FEMGPU_E_HIP, "PCG made no progress (kernels not running)");
-    last_iter = h.iter;
+    last_iter = h.iter_final;
     cur = nxt;
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->h_sc[0] = ctx->h_sc[(cur + 1) & 1];
-  *iters = ctx->h_sc->iter;
+  return FEMGPU_OK;
+}
+
+// Runs PCG until the stop flag is set.  r holds the initial residual; x and p
+// are cleared here.
+template <typename T>
+int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, int max_iter,
+            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code) {
+  double *part_patch, *part_node;
+  int st;
+  if ((st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)) ||
+      (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
+    return st;
+  const int64_t nd = ctx->nn * ctx->dim;
+  femgpu::launch_fill<T>(w.x, nd, T(0), ctx->stream);
+  femgpu::launch_fill<T>(w.p, nd, T(0), ctx->stream);
+  femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
+  femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.z, w.invD, sc, part_node, ctx->stream);
+  HIPCHK(hipGetLastError());
+  block = std::max(2, block + (block & 1));
+  hipGraphExec_t exec;
+  if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node, &exec)) ||
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0)))
+    return st;
+  *iters = ctx->h_sc->iter_final;
   *done_code = ctx->h_sc->done;
   return FEMGPU_OK;
 }
@@ -605,7 +625,14 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   dm.n_owned = P.n_owned;
   dm.n_iface = P.n_iface;
   dm.n_active = P.n_owned + P.n_iface;
-  dm.patch_threads = ctx->threads_req ? ctx->threads_req : 512;
+  dm.patch_threads = ctx->threads_req ? ctx->threads_req : 256;
+  // one workgroup thread handles at most two elements of its patch
+  if (dm.emax > 2 * dm.patch_threads) {
+    if (ctx->threads_req)
+      return fail(ctx, FEMGPU_E_ARG, "elements per patch exceed twice the patch threads");
+    dm.patch_threads = 512;
+  }
+  if (dm.emax > 1024) return fail(ctx, FEMGPU_E_ARG, "elements per patch above 1024");
   int st;
   femgpu::PatchInfo *pp;
   uint32_t *lid;
@@ -613,7 +640,7 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   uint16_t *ne_, *inc;
   if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "elem_lid", P.elem_lid, &lid)) ||
       (st = upload(ctx, "sh_gid", P.sh_gid, &shg)) || (st = upload(ctx, "sh_slot", P.sh_slot, &shs)) ||
-      (st = upload(ctx, "node_end", P.node_end, &ne_)) || (st = upload(ctx, "inc", P.inc, &inc)) ||
+      (st = upload(ctx, "node_end", P.node_end, &ne_)) || (st = upload(ctx, "epos", P.epos, &inc)) ||
       (st = upload(ctx, "slot_start", P.slot_start, &ss)) ||
       (st = upload(ctx, "elems_int", P.elems_int, &ei)))
     return st;
@@ -622,7 +649,7 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   dm.sh_gid = shg;
   dm.sh_slot = shs;
   dm.node_end = ne_;
-  dm.inc = inc;
+  dm.epos = inc;
   dm.slot_start = ss;
   dm.elems_int = ei;
   std::vector<double> c64(size_t(n_nodes) * dim);
@@ -650,6 +677,8 @@ int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads) {
     return fail(ctx, FEMGPU_E_ARG, "elems_per_patch out of range");
   if (patch_threads != 0 && patch_threads != 256 && patch_threads != 512)
     return fail(ctx, FEMGPU_E_ARG, "patch_threads must be 256 or 512");
+  if (patch_threads && ctx->have_mesh && ctx->dm.emax > 2 * patch_threads)
+    return fail(ctx, FEMGPU_E_ARG, "current mesh has patches larger than 2 x patch_threads");
   ctx->emax = elems_per_patch;
   ctx->threads_req = patch_threads;
   if (patch_threads) ctx->dm.patch_threads = patch_threads;
@@ -859,7 +888,7 @@ int femgpu_default_solve_opts(femgpu_solve_opts *o) {
   o->tol_kind = FEMGPU_TOL_REF_GAMMA;
   o->tol = 1e-8;
   o->max_iter = 0;
-  o->inner_tol = 1e-3;
+  o->inner_tol = 0.1;
   o->check_every = 32;
   return FEMGPU_OK;
 }
@@ -903,7 +932,7 @@ int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *
       return st;
     femgpu::launch_copy<double>(w.x, x64, int64_t(nd), ctx->stream);
     S.iterations = iters;
-    S.gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->scale / ctx->h_sc->gamma0 : 0.0;
+    S.gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->g_final / ctx->h_sc->gamma0 : 0.0;
     S.status = done_status(done);
   } else if (o.precision == FEMGPU_FP32) {
     PcgWs<float> w;
@@ -915,61 +944,74 @@ int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *
       return st;
     femgpu::launch_convert<float, double>(w.x, x64, int64_t(nd), ctx->stream);
     S.iterations = iters;
-    S.gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->scale / ctx->h_sc->gamma0 : 0.0;
+    S.gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->g_final / ctx->h_sc->gamma0 : 0.0;
     S.status = done_status(done);
   } else if (o.precision == FEMGPU_MIXED) {
-    // FP32 PCG inside FP64 iterative refinement: r = f - K x in FP64, solve
-    // K d = r / ||r|| in FP32 to inner_tol, x += ||r|| d.
+    // FP32 PCG (matvec, preconditioner, vectors) with FP64 dot products, the
+    // solution accumulated in FP64 and the residual replaced by the FP64
+    // residual f - K x64 whenever the FP32 residual has dropped by
+    // `inner_tol` (reliable updates), and at the end until the FP64 residual
+    // meets the tolerance.
     PcgWs<float> w;
-    double *r64, *y64, *part64, *part_node;
+    double *r64, *y64, *part64, *part_node, *part_patch;
     if ((st = pcg_ws(ctx, "c32", w)) || (st = dbuf(ctx, "ir_r", nd, &r64)) ||
         (st = dbuf(ctx, "ir_y", nd, &y64)) ||
         (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part64)) ||
-        (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
+        (st = dbuf(ctx, "part_node", NODE_PART, &part_node)) ||
+        (st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)))
       return st;
     w.invD = static_cast<float *>(ctx->bufs["invD32"].p);
+    const double drop = o.inner_tol > 0 && o.inner_tol < 1 ? o.inner_tol : 0.1;
+    const int block_m = std::max(2, block + (block & 1));
     femgpu::launch_fill<double>(x64, int64_t(nd), 0.0, ctx->stream);
-    femgpu::launch_fill<double>(y64, int64_t(nd), 0.0, ctx->stream);
-    femgpu::launch_reset_scalars(sc2, 1, 0.0, 0, ctx->stream);
-    // ||f||: residual of x = 0 (y = 0 everywhere, no partial slots read for
-    // owned nodes; interface slots cleared by a product of x = 0).
-    femgpu::launch_matvec_patch<double>(ctx->dm, x64, y64, part64, ctx->stream);
-    femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
-    if ((st = poll(ctx, sc2))) return st;
-    const double fnorm = std::sqrt(ctx->h_sc->rr);
-    double rel = fnorm > 0 ? 1.0 : 0.0;
-    int outer = 0;
-    int status = FEMGPU_OK;
-    const double floor_tol = o.inner_tol > 0 ? o.inner_tol : 1e-3;
-    while (rel > o.tol && outer < 50) {
-      femgpu::launch_to_f32_scaled(ctx->dm, r64, w.r, w.x, sc2, ctx->stream);
-      // ask the FP32 solve only for the reduction still missing (with a
-      // factor 2 of margin), but never below what FP32 can deliver
-      const double inner = std::min(0.5, std::max(floor_tol, 0.5 * o.tol / rel));
-      int it = 0;
-      if ((st = run_pcg<float>(ctx, w, OpShift<float>(), FEMGPU_TOL_REL_RES, inner,
-                               std::max(1, max_iter - iters), block, 1, sc, &it, &done)))
-        return st;
-      iters += it;
-      if (done == 2 || done == 3) {
+    femgpu::launch_fill<float>(w.x, int64_t(nd), 0.f, ctx->stream);
+    femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
+    femgpu::launch_reset_scalars(sc, FEMGPU_TOL_REL_RES, o.tol, max_iter, ctx->stream);
+    femgpu::launch_pcg_replace(ctx->dm, f64, w.r, w.z, w.invD, part_node, ctx->stream);
+    femgpu::launch_fin_init(sc, part_node, ctx->stream);
+    HIPCHK(hipGetLastError());
+    if ((st = poll(ctx, sc))) return st;
+    const double ff = ctx->h_sc->rr0;
+    hipGraphExec_t exec;
+    if ((st = pcg_graph<float>(ctx, w, OpShift<float>(), block_m, 1, sc, part_patch, part_node,
+                               &exec)))
+      return st;
+    int status = FEMGPU_OK, replacements = 0;
+    double pause = ff * drop * drop;
+    double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
+    while (rel > o.tol) {
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause))) return st;
+      const int done = ctx->h_sc->done;
+      iters = ctx->h_sc->iter_final;
+      femgpu::launch_flush_x(ctx->dm, x64, w.x, ctx->stream);
+      if (done != 0 && done != 1) {
         status = done_status(done);
         break;
       }
-      femgpu::launch_axpy_back(ctx->dm, x64, w.x, sc2, ctx->stream);
+      // FP64 residual of the accumulated solution
+      femgpu::launch_reset_scalars(sc2, 1, 0.0, 0, ctx->stream);
       femgpu::launch_matvec_patch<double>(ctx->dm, x64, y64, part64, ctx->stream);
       femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
       HIPCHK(hipGetLastError());
       if ((st = poll(ctx, sc2))) return st;
-      rel = fnorm > 0 ? std::sqrt(ctx->h_sc->rr) / fnorm : 0.0;
-      ++outer;
-      if (iters >= max_iter && rel > o.tol) {
+      rel = ff > 0 ? std::sqrt(ctx->h_sc->rr / ff) : 0.0;
+      if (rel <= o.tol) break;
+      if (iters >= max_iter || replacements > 1000) {
         status = FEMGPU_E_MAXIT;
         break;
       }
+      femgpu::launch_pcg_replace(ctx->dm, r64, w.r, w.z, w.invD, part_node, ctx->stream);
+      if (done == 1) femgpu::launch_pcg_resume(sc, ctx->stream);
+      // If the FP64 residual did not at least halve since the previous
+      // replacement, the FP32 directions have lost conjugacy: restart the
+      // Krylov space from the FP64 residual (p = 0), i.e. plain refinement.
+      if (rel > 0.5 * rel_prev) femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
+      rel_prev = rel;
+      pause = ctx->h_sc->rr * drop * drop;
+      ++replacements;
     }
-    if (status == FEMGPU_OK && rel > o.tol) status = FEMGPU_E_MAXIT;
     S.iterations = iters;
-    S.outer_iterations = outer;
+    S.outer_iterations = replacements;
     S.status = status;
   } else {
     return fail(ctx, FEMGPU_E_ARG, "unknown precision");

@@ -12,8 +12,10 @@
 //  * Per element: its patch-local node ids packed as uint8 in one uint32
 //    (npe <= 4, patch nodes <= 256).
 //  * Per patch node (owned first, then interface): the end offset of its
-//    incidence list; incidence entries are uint16 codes inst*npe + corner
-//    (sorted), stored per patch at offset npe*elem_start.
+//    run in the patch's node-sorted list of (element, corner) entries; per
+//    element and corner (stride 4): the uint16 position of that entry in the
+//    list, so an element writes its nodal forces straight to their sorted
+//    place and each node's sum is a contiguous fixed-order run.
 //  * Interface nodes: a partial-sum slot per (patch, node), slots of one node
 //    contiguous in patch order, so the final sum is a fixed-order loop.
 // Node vectors are AoS by DOF: v[node*dim + c].
@@ -57,7 +59,7 @@ struct MeshPlan {
   std::vector<int32_t> sh_gid;       // per patch interface node: internal id
   std::vector<int32_t> sh_slot;      // per patch interface node: slot
   std::vector<uint16_t> node_end;    // per patch node
-  std::vector<uint16_t> inc;         // npe per element, per patch segment
+  std::vector<uint16_t> epos;        // 4 per element: sorted entry position
   std::vector<int32_t> slot_start;   // n_iface + 1
   std::vector<int32_t> elems_int;    // ne x npe internal node ids, internal order
 };

@@ -22,7 +22,7 @@ struct DevMesh {
   const int32_t *sh_gid = nullptr;
   const int32_t *sh_slot = nullptr;
   const uint16_t *node_end = nullptr;
-  const uint16_t *inc = nullptr;
+  const uint16_t *epos = nullptr;
   const int32_t *slot_start = nullptr;
   const uint8_t *fixmask = nullptr;   // per internal node: bit c = component c fixed
   const int32_t *elems_int = nullptr; // ne x npe internal node ids
@@ -33,23 +33,27 @@ struct DevMesh {
   double lam = 0, mu = 0;             // Lame constants (2D: plane stress)
 };
 
-// Scalars of one PCG run, device resident.
+// Scalars of one PCG run, device resident.  Iteration j of the fused
+// iteration writes its gamma/beta/count into slot j&1 and reads slot
+// (j&1)^1, so no kernel reads a field that another block of the same launch
+// writes.
 struct CgScalars {
-  double gamma;        // current r'z
-  double gamma0;
-  double alpha;
-  double beta;
-  double pq;
-  double rr;           // r'r after the last update
-  double rr0;
+  double gamma0;       // r0'z0
+  double rr0;          // r0'r0
   double thresh;       // stop threshold (kind dependent)
   double eps;          // tol as given
+  double g[2];         // gamma_j = r_j'z_j
+  double b[2];         // beta_j
+  double g_final;      // last gamma seen by the stop test
+  double rr_final;     // last r'r seen by the stop test
+  double pq;           // last p'q (diagnostic)
+  double rr;           // r'r of the FP64 refinement residual
+  double scale;        // sqrt(rr) (refinement scaling)
+  int it[2];           // updates done before iteration j
+  int iter_final;      // updates done when the stop flag was set
   int kind;            // 0: gamma ratio, 1: relative residual
   int done;            // 0 running, 1 converged, 2 non-finite, 3 breakdown, 4 max-iter
-  int iter;
   int max_iter;
-  int pad0, pad1;
-  double scale;        // scratch scalar (norms for refinement)
 };
 
 // Optional mass shift of the operator: y = cK*K x + cM*diag(m) x.
@@ -80,21 +84,27 @@ void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T
 // PCG: init (z = invD r, gamma0) and one fused iteration.
 template <typename T>
 void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScalars *sc,
-                     double *part, hipStream_t s);
+                     double *part_node, hipStream_t s);
+// parity: j & 1 of the iteration (ping-pong of scalars and partial buffers).
+// part_patch: >= np doubles; part_node: 4 * NODE_PARTS doubles.
 template <typename T>
 void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial,
-                     const T *invD, OpShift<T> sh, CgScalars *sc, double *part_a,
-                     double *part_b, hipStream_t s);
+                     const T *invD, OpShift<T> sh, CgScalars *sc, double *part_patch,
+                     double *part_node, int parity, hipStream_t s);
+constexpr int NODE_PARTS = 256;  // blocks of the node-parallel PCG kernels
 
 // r = f - (interface-finished) K x, rr -> sc->rr (FP64 outer loop of MIXED).
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
                        double *r, CgScalars *sc, double *part, hipStream_t s);
-// r32 = r64 / sqrt(sc->rr); x32 = 0.
-void launch_to_f32_scaled(const DevMesh &dm, const double *r64, float *r32, float *x32,
-                          const CgScalars *sc, hipStream_t s);
-// x64 += sqrt(sc->rr) * x32.
-void launch_axpy_back(const DevMesh &dm, double *x64, const float *x32, const CgScalars *sc,
-                      hipStream_t s);
+// Mixed solve (reliable updates): x64 += x32, x32 = 0; residual replacement
+// r32 = r64, z32 = invD r32 with the partials the next iteration reads;
+// resume after a stop flag.
+void launch_flush_x(const DevMesh &dm, double *x64, float *x32, hipStream_t s);
+void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, float *z32,
+                        const float *invD, double *part_node, hipStream_t s);
+void launch_pcg_resume(CgScalars *sc, hipStream_t s);
+// gamma0 / thresholds from the slot-1 partials (after launch_pcg_replace).
+void launch_fin_init(CgScalars *sc, double *part_node, hipStream_t s);
 
 template <typename T>
 void launch_fill(T *p, int64_t n, T v, hipStream_t s);

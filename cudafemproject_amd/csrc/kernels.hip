@@ -36,7 +36,7 @@ constexpr int NT = 256;          // threads per block for every kernel
 constexpr int NODE_GRID_MAX = 2048;
 
 struct LdsLayout {
-  int off_c, off_x, off_res, off_end, off_inc, off_fix, off_red, total;
+  int off_c, off_x, off_res, off_end, off_fix, off_red, total;
 };
 
 __host__ __device__ inline int al16(int v) { return (v + 15) & ~15; }
@@ -48,8 +48,7 @@ __host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ema
   L.off_x = al16(pn * dim * tb);
   L.off_res = L.off_x + al16(pn * dim * tb);
   L.off_end = L.off_res + al16(emax * nde * tb);
-  L.off_inc = L.off_end + al16(pn * 2);
-  L.off_fix = L.off_inc + al16(emax * 4 * 2);
+  L.off_fix = L.off_end + al16(pn * 2);
   L.off_red = L.off_fix + al16(pn);
   L.total = L.off_red + 16 * 8;
   return L;
@@ -252,8 +251,9 @@ __device__ __forceinline__ void elem_diag(const T *X, T lam, T mu, T es, T *F) {
 }
 
 // ---------------------------------------------------------------------------
-// Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step
-// (p = z + beta p, q = cK K p + cM m p, p'q reduced to alpha).  MODE 2: diag(K).
+// Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step (stop test and beta
+// from the previous update's partial sums, p = z + beta p, q = cK K p + cM m p,
+// per-patch p'q partial).  MODE 2: diag(K).
 
 template <typename T>
 struct PatchArgs {
@@ -262,7 +262,7 @@ struct PatchArgs {
   const int32_t *sh_gid;
   const int32_t *sh_slot;
   const uint16_t *node_end;
-  const uint16_t *inc;
+  const uint16_t *epos;
   const uint8_t *fixmask;
   const int32_t *slot_start;
   const T *coords;
@@ -276,7 +276,10 @@ struct PatchArgs {
   T *partial;   // interface slots
   OpShift<T> sh;
   CgScalars *sc;
-  double *part;
+  double *part;       // MODE 1: per-patch p'q partials
+  const double *prz;  // MODE 1: r'z partials of the previous update (NODE_PARTS)
+  const double *prr;  // MODE 1: r'r partials of the previous update
+  int parity;         // MODE 1: j & 1
 };
 
 template <typename T, int DIM, int MODE, int PT>
@@ -288,22 +291,73 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   // interface nodes) go to the same XCD and its L2.
   const int pid = (b & 7) * a.p8 + (b >> 3);
   if (pid >= a.np) return;
-  if (MODE == 1 && a.sc->done) return;
   const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.emax);
   T *s_c = reinterpret_cast<T *>(lds + L.off_c);
   T *s_x = reinterpret_cast<T *>(lds + L.off_x);
-  T *s_res = reinterpret_cast<T *>(lds + L.off_res);
+  T *s_res = reinterpret_cast<T *>(lds + L.off_res);   // node-sorted entries x DIM
   uint16_t *s_end = reinterpret_cast<uint16_t *>(lds + L.off_end);
-  uint16_t *s_inc = reinterpret_cast<uint16_t *>(lds + L.off_inc);
   uint8_t *s_fix = reinterpret_cast<uint8_t *>(lds + L.off_fix);
   double *s_red = reinterpret_cast<double *>(lds + L.off_red);
 
+  T beta = T(0);
+  if constexpr (MODE == 1) {
+    CgScalars *sc = a.sc;
+    if (sc->done) return;
+    // Reduce the previous update's r'z and r'r (fixed order, every block
+    // gets the same values), run the stop test, form beta.
+    double g = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < NODE_PARTS; i += PT) {
+      g += a.prz[i];
+      q += a.prr[i];
+    }
+    g = block_sum<PT>(g, s_red);
+    q = block_sum<PT>(q, s_red);
+    const int q1 = a.parity ^ 1;
+    const int n_upd = sc->it[q1] + 1;
+    int stop = 0;
+    if (n_upd >= 1) {
+      const bool conv = sc->kind == 0 ? (g < sc->thresh) : (q <= sc->thresh);
+      if (conv) stop = 1;
+    } else if (q == 0.0) {
+      stop = 1;
+    }
+    if (!stop && !isfinite(g)) stop = 2;
+    if (!stop && n_upd >= sc->max_iter) stop = 4;
+    const double bt = g / sc->g[q1];
+    if (pid == 0 && threadIdx.x == 0) {
+      sc->g_final = g;
+      sc->rr_final = q;
+      sc->iter_final = n_upd;
+      if (stop) {
+        sc->done = stop;
+      } else {
+        sc->g[a.parity] = g;
+        sc->b[a.parity] = bt;
+        sc->it[a.parity] = n_upd;
+      }
+    }
+    if (stop) return;
+    beta = T(bt);
+  }
   const PatchInfo P = a.patches[pid];
   const int nloc = P.n_own + P.n_sh;
   double acc_en = 0.0;
-  const T beta = MODE == 1 ? T(a.sc->beta) : T(0);
   const T cK = MODE == 1 ? a.sh.cK : T(1);
   const T cM = MODE == 1 ? a.sh.cM : T(0);
+
+  // Prefetch this thread's element ids and entry positions (phase B) so
+  // their latency overlaps the node staging.
+  constexpr int EPT = 2;  // elements per thread (emax <= 2*PT)
+  uint32_t lid_pf[EPT];
+  uint2 pos_pf[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * PT;
+    if (i < P.n_elem) {
+      lid_pf[e] = a.elem_lid[P.elem_start + i];
+      pos_pf[e] = reinterpret_cast<const uint2 *>(a.epos)[P.elem_start + i];
+    }
+  }
 
   // Phase A: stage the patch's nodes in LDS.
   for (int k = threadIdx.x; k < nloc; k += PT) {
@@ -336,16 +390,18 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
         }
       }
     }
-    s_fix[k] = a.fixmask[gid];
+    if constexpr (MODE != 2) s_fix[k] = a.fixmask[gid];
     s_end[k] = a.node_end[P.pnode_start + k];
   }
-  for (int j = threadIdx.x; j < P.n_elem * NPE; j += PT)
-    s_inc[j] = a.inc[int64_t(P.elem_start) * NPE + j];
   __syncthreads();
 
-  // Phase B: one element per thread, forces into LDS.
-  for (int i = threadIdx.x; i < P.n_elem; i += PT) {
-    const uint32_t lid = a.elem_lid[P.elem_start + i];
+  // Phase B: one element per thread, nodal forces written to their
+  // node-sorted entry positions.
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * PT;
+    if (i >= P.n_elem) break;
+    const uint32_t lid = lid_pf[e];
     T X[NDE], U[NDE];
     unsigned fix = 0;
 #pragma unroll
@@ -366,23 +422,37 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
       const T en = elem_force<T, DIM>(X, U, a.lam, a.mu, es, fix, F);
       if constexpr (MODE == 1) acc_en += double(cK) * double(en);
     }
+    const unsigned pos[4] = {pos_pf[e].x & 0xffffu, pos_pf[e].x >> 16, pos_pf[e].y & 0xffffu,
+                             pos_pf[e].y >> 16};
 #pragma unroll
-    for (int j = 0; j < NDE; ++j) s_res[j * a.emax + i] = F[j];
+    for (int n = 0; n < NPE; ++n)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) s_res[pos[n] * DIM + c] = F[n * DIM + c];
   }
   __syncthreads();
 
-  // Phase C: fixed-order node sums; owned nodes final, interface to slots.
+  // Phase C: each node sums its contiguous run of entries in order; owned
+  // nodes are final, interface nodes go to their partial slot.
   for (int k = threadIdx.x; k < nloc; k += PT) {
     const int b0 = k ? int(s_end[k - 1]) : 0, b1 = int(s_end[k]);
     T acc[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) acc[c] = T(0);
-    for (int j = b0; j < b1; ++j) {
-      const int code = s_inc[j];
-      const int inst = code >> 2, corner = code & 3;
+    int j = b0;
+    for (; j + 4 <= b1; j += 4) {
+      T v[4][DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) acc[c] += s_res[(corner * DIM + c) * a.emax + inst];
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) v[u][c] = s_res[(j + u) * DIM + c];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] += v[u][c];
     }
+    for (; j < b1; ++j)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) acc[c] += s_res[j * DIM + c];
     if (k < P.n_own) {
       const int64_t gid = P.own_start + k;
 #pragma unroll
@@ -437,17 +507,40 @@ struct UpdArgs {
   const T *invD;
   OpShift<T> sh;
   CgScalars *sc;
-  double *part_a, *part_b;
+  const double *ppq;  // per-patch p'q partials of this iteration
+  int np;
+  double *prz, *prr;  // this iteration's r'z, r'r partials (NODE_PARTS each)
+  int parity;
 };
 
+constexpr int UPT = 512;  // threads per block of the node-parallel PCG kernels
+
+// Fused PCG update (femfunc.cpp:66-81): alpha from the per-patch p'q
+// partials (every block reduces them in the same fixed order), finish the
+// interface nodes of q (and p), x += alpha p, r -= alpha q, z = invD r, and
+// per-block r'z, r'r partials for the next iteration's stop test.
 template <typename T, int DIM>
-__global__ __launch_bounds__(NT) void k_pcg_update(UpdArgs<T> a) {
-  __shared__ double s_red[NT / 64];
-  if (a.sc->done) return;
-  const T alpha = T(a.sc->alpha), beta = T(a.sc->beta);
+__global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
+  __shared__ double s_red[UPT / 64];
+  CgScalars *sc = a.sc;
+  if (sc->done) return;
+  double pq = 0.0;
+  for (int i = threadIdx.x; i < a.np; i += UPT) pq += a.ppq[i];
+  pq = block_sum<UPT>(pq, s_red);
+  const double gam = sc->g[a.parity];
+  const double alpha_d = gam / pq;
+  if (!(pq > 0.0) || !isfinite(alpha_d)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->done = isfinite(pq) ? 3 : 2;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq = pq;
+  const T alpha = T(alpha_d), beta = T(sc->b[a.parity]);
   double rz = 0.0, rr = 0.0;
-  const int64_t stride = int64_t(gridDim.x) * NT;
-  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < a.n_active; i += stride) {
+  const int64_t stride = int64_t(gridDim.x) * UPT;
+  for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < a.n_active; i += stride) {
     T qv[DIM], pv[DIM];
     if (i < a.n_owned) {
 #pragma unroll
@@ -481,63 +574,39 @@ __global__ __launch_bounds__(NT) void k_pcg_update(UpdArgs<T> a) {
       rr += double(rv) * double(rv);
     }
   }
-  rz = block_sum(rz, s_red);
-  rr = block_sum(rr, s_red);
+  rz = block_sum<UPT>(rz, s_red);
+  rr = block_sum<UPT>(rr, s_red);
   if (threadIdx.x == 0) {
-    a.part_a[blockIdx.x] = rz;
-    a.part_b[blockIdx.x] = rr;
+    a.prz[blockIdx.x] = rz;
+    a.prr[blockIdx.x] = rr;
   }
 }
 
-// Finalize of the p'q reduction (one block): alpha = gamma / p'q.
-__global__ __launch_bounds__(NT) void k_fin_alpha(CgScalars *sc, const double *part, int n) {
-  __shared__ double s_red[NT / 64];
-  if (sc->done) return;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) v += part[i];
-  v = block_sum(v, s_red);
+// PCG initialisation: z = invD r and the r'z, r'r partials that the first
+// fused iteration (parity 0) reads from slot 1.
+template <typename T>
+__global__ __launch_bounds__(UPT) void k_pcg_init(int64_t n, const T *r, T *z, const T *invD,
+                                                  double *prz, double *prr) {
+  __shared__ double s_red[UPT / 64];
+  double rz = 0.0, rr = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * UPT;
+  for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < n; i += stride) {
+    const T rv = r[i];
+    const T zv = invD[i] * rv;
+    z[i] = zv;
+    rz += double(rv) * double(zv);
+    rr += double(rv) * double(rv);
+  }
+  rz = block_sum<UPT>(rz, s_red);
+  rr = block_sum<UPT>(rr, s_red);
   if (threadIdx.x == 0) {
-    sc->pq = v;
-    const double alpha = sc->gamma / v;
-    if (!(v > 0.0) || !isfinite(alpha)) {
-      sc->alpha = 0.0;
-      sc->done = isfinite(v) ? 3 : 2;
-    } else {
-      sc->alpha = alpha;
-    }
+    prz[blockIdx.x] = rz;
+    prr[blockIdx.x] = rr;
   }
 }
 
-// Finalize of the r'z, r'r reductions (one block): stop test, beta
-// (femfunc.cpp:81-110 order: test gamma_new first, then beta, then NaN check).
-__global__ __launch_bounds__(NT) void k_fin_beta(CgScalars *sc, const double *pa, const double *pb,
-                                                 int n) {
-  __shared__ double s_red[NT / 64];
-  if (sc->done) return;
-  double g = 0.0, q = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) {
-    g += pa[i];
-    q += pb[i];
-  }
-  g = block_sum(g, s_red);
-  q = block_sum(q, s_red);
-  if (threadIdx.x == 0) {
-    sc->iter += 1;
-    sc->rr = q;
-    sc->scale = g;  // last gamma_new
-    const bool stop = sc->kind == 0 ? (g < sc->thresh) : (q <= sc->thresh);
-    if (stop) {
-      sc->done = 1;
-    } else {
-      sc->beta = g / sc->gamma;
-      sc->gamma = g;
-      if (!isfinite(g) || !isfinite(sc->beta)) sc->done = 2;
-    }
-    if (!sc->done && sc->iter >= sc->max_iter) sc->done = 4;
-  }
-}
-
-// Finalize of the PCG initialisation (one block).
+// Scalars of the initial residual (one block): gamma0, thresholds, and the
+// slot-1 values the first iteration reads (beta_0 is irrelevant: p = 0).
 __global__ __launch_bounds__(NT) void k_fin_init(CgScalars *sc, const double *pa, const double *pb,
                                                  int n) {
   __shared__ double s_red[NT / 64];
@@ -550,15 +619,15 @@ __global__ __launch_bounds__(NT) void k_fin_init(CgScalars *sc, const double *pa
   q = block_sum(q, s_red);
   if (threadIdx.x == 0) {
     sc->gamma0 = g;
-    sc->gamma = g;
-    sc->scale = g;
     sc->rr0 = q;
-    sc->rr = q;
-    sc->alpha = 0.0;
-    sc->beta = 0.0;
-    sc->iter = 0;
+    sc->g_final = g;
+    sc->rr_final = q;
+    sc->g[0] = sc->g[1] = 1.0;
+    sc->b[0] = sc->b[1] = 0.0;
+    sc->it[0] = sc->it[1] = -1;
+    sc->iter_final = 0;
     sc->thresh = sc->kind == 0 ? sc->eps * g : sc->eps * sc->eps * q;
-    sc->done = q == 0.0 ? 1 : (isfinite(g) ? 0 : 2);
+    sc->done = isfinite(g) ? 0 : 2;
   }
 }
 
@@ -574,37 +643,18 @@ __global__ __launch_bounds__(NT) void k_fin_norm(CgScalars *sc, const double *pa
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_pcg_init(int64_t n, const T *r, T *z, const T *invD,
-                                                 CgScalars *sc, double *part_a, double *part_b) {
-  __shared__ double s_red[NT / 64];
-  double rz = 0.0, rr = 0.0;
-  const int64_t stride = int64_t(gridDim.x) * NT;
-  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
-    const T rv = r[i];
-    const T zv = invD[i] * rv;
-    z[i] = zv;
-    rz += double(rv) * double(zv);
-    rr += double(rv) * double(rv);
-  }
-  rz = block_sum(rz, s_red);
-  rr = block_sum(rr, s_red);
-  if (threadIdx.x == 0) {
-    part_a[blockIdx.x] = rz;
-    part_b[blockIdx.x] = rr;
-  }
-}
-
 __global__ void k_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sc->gamma = sc->gamma0 = sc->alpha = sc->beta = sc->pq = sc->rr = sc->rr0 = 0.0;
-    sc->thresh = 0.0;
+    sc->gamma0 = sc->rr0 = sc->thresh = 0.0;
+    sc->g[0] = sc->g[1] = 1.0;
+    sc->b[0] = sc->b[1] = 0.0;
+    sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = 0.0;
+    sc->it[0] = sc->it[1] = -1;
+    sc->iter_final = 0;
     sc->eps = eps;
     sc->kind = kind;
     sc->done = 0;
-    sc->iter = 0;
     sc->max_iter = max_iter;
-    sc->scale = 0.0;
   }
 }
 
@@ -641,22 +691,52 @@ __global__ __launch_bounds__(NT) void k_residual64(int64_t n_owned, int64_t n_ac
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
-__global__ __launch_bounds__(NT) void k_to_f32_scaled(int64_t n, const double *r64, float *r32,
-                                                      float *x32, const CgScalars *sc) {
-  const double s = sc->scale > 0.0 ? 1.0 / sc->scale : 0.0;
+// Reliable-update step of the mixed solve: x64 += x32, x32 = 0.
+__global__ __launch_bounds__(NT) void k_flush_x(int64_t n, double *x64, float *x32) {
   const int64_t stride = int64_t(gridDim.x) * NT;
   for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
-    r32[i] = float(r64[i] * s);
+    x64[i] += double(x32[i]);
     x32[i] = 0.0f;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_axpy_back(int64_t n, double *x64, const float *x32,
-                                                  const CgScalars *sc) {
-  const double s = sc->scale;
-  const int64_t stride = int64_t(gridDim.x) * NT;
-  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride)
-    x64[i] += s * double(x32[i]);
+// Residual replacement: r32 = r64 (the FP64 residual f - K x64), z32 =
+// invD r32, and the r'z, r'r partials the next fused iteration (parity 0)
+// reads from slot 1.
+__global__ __launch_bounds__(UPT) void k_pcg_replace(int64_t n, const double *r64, float *r32,
+                                                     float *z32, const float *invD, double *prz,
+                                                     double *prr) {
+  __shared__ double s_red[UPT / 64];
+  double rz = 0.0, rr = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * UPT;
+  for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < n; i += stride) {
+    const float rv = float(r64[i]);
+    const float zv = invD[i] * rv;
+    r32[i] = rv;
+    z32[i] = zv;
+    rz += double(rv) * double(zv);
+    rr += double(rv) * double(rv);
+  }
+  rz = block_sum<UPT>(rz, s_red);
+  rr = block_sum<UPT>(rr, s_red);
+  if (threadIdx.x == 0) {
+    prz[blockIdx.x] = rz;
+    prr[blockIdx.x] = rr;
+  }
+}
+
+// Resume a stopped PCG at the next even iteration: the iteration that set
+// the stop flag had parity iter_final & 1 and left the last completed
+// iteration's gamma and count in the other slot; the next replay reads slot 1.
+__global__ void k_pcg_resume(CgScalars *sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (sc->iter_final & 1) {
+      sc->g[1] = sc->g[0];
+      sc->b[1] = sc->b[0];
+      sc->it[1] = sc->it[0];
+    }
+    sc->done = 0;
+  }
 }
 
 template <typename T>
@@ -870,7 +950,7 @@ PatchArgs<T> patch_args(const DevMesh &dm) {
   a.sh_gid = dm.sh_gid;
   a.sh_slot = dm.sh_slot;
   a.node_end = dm.node_end;
-  a.inc = dm.inc;
+  a.epos = dm.epos;
   a.fixmask = dm.fixmask;
   a.slot_start = dm.slot_start;
   if constexpr (sizeof(T) == 8) {
@@ -972,18 +1052,17 @@ void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T
 
 template <typename T>
 void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScalars *sc,
-                     double *part, hipStream_t s) {
+                     double *part_node, hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
-  const int g = grid_for(n);
-  hipLaunchKernelGGL((k_pcg_init<T>), dim3(g), dim3(NT), 0, s, n, r, z, invD, sc, part,
-                     part + NODE_GRID_MAX);
-  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, part, part + NODE_GRID_MAX, g);
+  double *prz = part_node + 2 * NODE_PARTS, *prr = part_node + 3 * NODE_PARTS;  // slot 1
+  hipLaunchKernelGGL((k_pcg_init<T>), dim3(NODE_PARTS), dim3(UPT), 0, s, n, r, z, invD, prz, prr);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, prz, prr, NODE_PARTS);
 }
 
 template <typename T>
 void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial, const T *invD,
-                     OpShift<T> sh, CgScalars *sc, double *part_a, double *part_b,
-                     hipStream_t s) {
+                     OpShift<T> sh, CgScalars *sc, double *part_patch, double *part_node,
+                     int parity, hipStream_t s) {
   PatchArgs<T> a = patch_args<T>(dm);
   a.x = z;
   a.p = p;
@@ -991,12 +1070,14 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial
   a.partial = partial;
   a.sh = sh;
   a.sc = sc;
-  a.part = part_a;
+  a.part = part_patch;
+  a.prz = part_node + (2 * (parity ^ 1) + 0) * NODE_PARTS;
+  a.prr = part_node + (2 * (parity ^ 1) + 1) * NODE_PARTS;
+  a.parity = parity;
   if (dm.dim == 3)
     launch_patch<T, 3, 1>(dm, a, s);
   else
     launch_patch<T, 2, 1>(dm, a, s);
-  hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, sc, part_a, dm.np);
   UpdArgs<T> u{};
   u.n_owned = dm.n_owned;
   u.n_active = dm.n_active;
@@ -1010,14 +1091,15 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial
   u.invD = invD;
   u.sh = sh;
   u.sc = sc;
-  u.part_a = part_b;
-  u.part_b = part_b + NODE_GRID_MAX;
-  const int g = grid_for(dm.n_active);
+  u.ppq = part_patch;
+  u.np = dm.np;
+  u.prz = part_node + (2 * parity + 0) * NODE_PARTS;
+  u.prr = part_node + (2 * parity + 1) * NODE_PARTS;
+  u.parity = parity;
   if (dm.dim == 3)
-    hipLaunchKernelGGL((k_pcg_update<T, 3>), dim3(g), dim3(NT), 0, s, u);
+    hipLaunchKernelGGL((k_pcg_update<T, 3>), dim3(NODE_PARTS), dim3(UPT), 0, s, u);
   else
-    hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(g), dim3(NT), 0, s, u);
-  hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, sc, u.part_a, u.part_b, g);
+    hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(NODE_PARTS), dim3(UPT), 0, s, u);
 }
 
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
@@ -1032,16 +1114,25 @@ void launch_residual64(const DevMesh &dm, const double *f, const double *y, cons
   hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(NT), 0, s, sc, part, g);
 }
 
-void launch_to_f32_scaled(const DevMesh &dm, const double *r64, float *r32, float *x32,
-                          const CgScalars *sc, hipStream_t s) {
+void launch_flush_x(const DevMesh &dm, double *x64, float *x32, hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
-  hipLaunchKernelGGL(k_to_f32_scaled, dim3(grid_for(n)), dim3(NT), 0, s, n, r64, r32, x32, sc);
+  hipLaunchKernelGGL(k_flush_x, dim3(grid_for(n)), dim3(NT), 0, s, n, x64, x32);
 }
 
-void launch_axpy_back(const DevMesh &dm, double *x64, const float *x32, const CgScalars *sc,
-                      hipStream_t s) {
+void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, float *z32,
+                        const float *invD, double *part_node, hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
-  hipLaunchKernelGGL(k_axpy_back, dim3(grid_for(n)), dim3(NT), 0, s, n, x64, x32, sc);
+  hipLaunchKernelGGL(k_pcg_replace, dim3(NODE_PARTS), dim3(UPT), 0, s, n, r64, r32, z32, invD,
+                     part_node + 2 * NODE_PARTS, part_node + 3 * NODE_PARTS);
+}
+
+void launch_fin_init(CgScalars *sc, double *part_node, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, part_node + 2 * NODE_PARTS,
+                     part_node + 3 * NODE_PARTS, NODE_PARTS);
+}
+
+void launch_pcg_resume(CgScalars *sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_resume, dim3(1), dim3(64), 0, s, sc);
 }
 
 template <typename T>
@@ -1141,7 +1232,8 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_pcg_init<T>(const DevMesh &, const T *, T *, const T *, CgScalars *,    \
                                    double *, hipStream_t);                                     \
   template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, T *, const T *,   \
-                                   OpShift<T>, CgScalars *, double *, double *, hipStream_t);  \
+                                   OpShift<T>, CgScalars *, double *, double *, int,           \
+                                   hipStream_t);                                               \
   template void launch_fill<T>(T *, int64_t, T, hipStream_t);                                  \
   template void launch_copy<T>(const T *, T *, int64_t, hipStream_t);                          \
   template void launch_stress<T>(const DevMesh &, const T *, T *, hipStream_t);                \

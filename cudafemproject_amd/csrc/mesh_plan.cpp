@@ -169,10 +169,10 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
   plan.n_slots = plan.slot_start[plan.n_iface];
   std::vector<int32_t> slot_next(plan.slot_start.begin(), plan.slot_start.end() - 1);
 
-  // 6. Per-patch local numbering, packed ids and incidence lists.
+  // 6. Per-patch local numbering, packed ids and node-sorted entry positions.
   plan.patches.resize(np);
   plan.elem_lid.assign(ne, 0);
-  plan.inc.assign(size_t(ne) * npe, 0);
+  plan.epos.assign(size_t(ne) * 4, 0);
   plan.elems_int.assign(size_t(ne) * npe, 0);
   std::vector<int32_t> local(nn, -1);
   std::vector<int32_t> loc_nodes;
@@ -233,7 +233,8 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
       }
       plan.elem_lid[i] = packed;
     }
-    // end offsets, then fill (entries sorted by instance, then corner)
+    // end offsets, then each (element, corner) entry's position in the
+    // node-sorted order (within a node: by element, then corner)
     std::vector<int32_t> pos(nloc);
     int32_t run = 0;
     for (int32_t k = 0; k < nloc; ++k) {
@@ -241,12 +242,11 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
       run += cnt[k];
       plan.node_end.push_back(uint16_t(run));
     }
-    uint16_t *seg = &plan.inc[size_t(P.elem_start) * npe];
     for (int32_t i = 0; i < P.n_elem; ++i) {
       uint32_t packed = plan.elem_lid[P.elem_start + i];
       for (int a = 0; a < npe; ++a) {
         int32_t l = int32_t((packed >> (8 * a)) & 255u);
-        seg[pos[l]++] = uint16_t(i * 4 + a);
+        plan.epos[size_t(P.elem_start + i) * 4 + a] = uint16_t(pos[l]++);
       }
     }
     for (int32_t v : loc_nodes) local[v] = -1;

@@ -48,7 +48,9 @@ enum {
 enum {
   FEMGPU_FP64 = 0,   /* FP64 vectors, FP64 EbE matvec                        */
   FEMGPU_FP32 = 1,   /* FP32 vectors and matvec, FP64 dot accumulation       */
-  FEMGPU_MIXED = 2   /* FP32 PCG inside FP64 iterative refinement            */
+  FEMGPU_MIXED = 2   /* FP32 PCG inside FP64 iterative refinement: FP64
+                        solution, FP64 residual replacement (reliable
+                        updates), FP64 dot products                          */
 };
 
 /* Stopping rules. */
@@ -153,15 +155,15 @@ typedef struct {
   int tol_kind;       /* FEMGPU_TOL_REF_GAMMA / FEMGPU_TOL_REL_RES            */
   double tol;         /* 1e-8 in the reference's static path (femfunc.cpp:20) */
   int max_iter;       /* total PCG iteration cap; <= 0: 1,000,000             */
-  double inner_tol;   /* MIXED: smallest relative residual reduction asked of
-                         one FP32 inner solve (default 1e-3)                  */
+  double inner_tol;   /* MIXED: drop of the FP32 residual norm after which it
+                         is replaced by the FP64 residual (default 0.1)       */
   int check_every;    /* iterations between host polls of the stop flag       */
 } femgpu_solve_opts;
 
 typedef struct {
   int status;          /* FEMGPU_OK or an error code                          */
   int iterations;      /* PCG iterations (summed over inner solves in MIXED)  */
-  int outer_iterations;/* MIXED: refinement steps                             */
+  int outer_iterations;/* MIXED: FP64 residual replacements                   */
   double rel_res;      /* true ||f - K u|| / ||f|| recomputed in FP64         */
   double gamma_ratio;  /* final gamma/gamma0 of the (last) PCG                */
   double solve_ms;     /* device time of the solve, setup excluded            */
