@@ -53,6 +53,24 @@ float wavelet_value(float t, int type, float freq, float timeshift, float ampl) 
   return 0.f;
 }
 
+// The same wavelets in double (FP64 state).
+double wavelet_value_f64(double t, int type, double freq, double timeshift, double ampl) {
+  const double PI = 3.14159265358979323846;
+  if (t < timeshift) return 0.0;
+  t = t - timeshift;
+  if (type == FEMGPU_WAVELET_RICKER) {
+    const double tmp = PI * freq * t - PI;
+    return ampl * (1.0 - 2.0 * tmp * tmp) * std::exp(-tmp * tmp);
+  }
+  if (type == FEMGPU_WAVELET_BERLAGE) {
+    const double w0 = 2 * PI * freq, w1 = w0 / std::sqrt(3.0);
+    return ampl * w1 * w1 / 4.0 * std::exp(-w1 * t) *
+           (std::sin(w0 * t) * (1.0 / (w1 * w1 * w1) + t / (w1 * w1) - t * t / w1) -
+            std::cos(w0 * t) * std::sqrt(3.0) * (t * t / w1 + t / (w1 * w1)));
+  }
+  return 0.0;
+}
+
 }  // namespace
 
 struct femgpu_ctx {
@@ -518,6 +536,7 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   for (auto &e : ctx->ev_poll) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
+  HIPCHK(femgpu::prepare_kernels());
   *out = c.release();
   return FEMGPU_OK;
 }
@@ -1138,24 +1157,32 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   const bool expl = o.beta2 == 0.0;
   int st;
   if ((st = ensure_diag(ctx))) return st;
-  // Time grid and RHS coefficients follow the reference's float arithmetic:
-  // endnt = int(endtime/dt), t = nt*dt, pressures overwritten by the wavelet
-  // value once it is non-zero (cpu_fem_utils.cpp:564-569).
-  const float dtf = float(o.dt), endf = float(o.endtime);
-  const int endnt = endf < 0.f ? 1 : int(endf / dtf);
+  // Time grid, GN22 coefficients and wavelet values follow the reference
+  // (endnt = int(endtime/dt), t = nt*dt, pressures overwritten by the wavelet
+  // value once it is non-zero, cpu_fem_utils.cpp:564-569), in float for FP32
+  // state as the reference computes them and in double for FP64 state.
+  const bool f32 = sizeof(T) == sizeof(float);
+  const double dtd = f32 ? double(float(o.dt)) : o.dt;
+  const int endnt = o.endtime < 0 ? 1
+                    : f32 ? int(float(o.endtime) / float(o.dt)) : int(o.endtime / o.dt);
   if (endnt < 1) return fail(ctx, FEMGPU_E_ARG, "endtime/dt gives no step");
   std::vector<double> cfile(endnt + 1, 1.0), cunit(endnt + 1, 0.0);
   {
     const bool loads = ctx->conv == FEMGPU_LOADS_REFERENCE && !ctx->lnode.empty();
-    const float f = float(o.wavelet_freq), ts = float(o.wavelet_timeshift), am = float(o.wavelet_ampl);
-    float last = wavelet_value(0.f, o.wavelet_type, f, ts, am);
+    auto wav = [&](int nt) -> double {
+      if (f32)
+        return double(wavelet_value(float(nt) * float(o.dt), o.wavelet_type, float(o.wavelet_freq),
+                                    float(o.wavelet_timeshift), float(o.wavelet_ampl)));
+      return wavelet_value_f64(double(nt) * o.dt, o.wavelet_type, o.wavelet_freq,
+                               o.wavelet_timeshift, o.wavelet_ampl);
+    };
+    double last = wav(0);
     for (int nt = 1; nt <= endnt; ++nt) {
-      const float t = float(nt) * dtf;
-      const float v = wavelet_value(t, o.wavelet_type, f, ts, am);
-      if (v != 0.f) last = v;
-      if (!loads && last != 0.f) {
+      const double v = wav(nt);
+      if (v != 0.0) last = v;
+      if (!loads && last != 0.0) {
         cfile[nt] = 0.0;
-        cunit[nt] = double(last);
+        cunit[nt] = last;
       }
     }
   }
@@ -1188,38 +1215,56 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   femgpu::launch_fill<T>(dv, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(da, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(dy, int64_t(nd), T(0), ctx->stream);
-  // GN22 coefficients in float as the reference forms them (femfunc.cpp:258-310).
-  const float b1 = float(o.beta1), b2 = float(o.beta2);
+  // GN22 coefficients (femfunc.cpp:258-310).
   femgpu::DynCoeffs C;
-  C.dt = double(dtf);
-  C.c1 = double(0.5f * (1.f - b2) * dtf * dtf);
-  C.c2 = double((1.f - b1) * dtf);
-  C.c3 = double(b1 * dtf);
-  C.c4 = double(0.5f * b2 * dtf * dtf);
+  if (f32) {
+    const float b1 = float(o.beta1), b2 = float(o.beta2), dtf = float(o.dt);
+    C.dt = double(dtf);
+    C.c1 = double(0.5f * (1.f - b2) * dtf * dtf);
+    C.c2 = double((1.f - b1) * dtf);
+    C.c3 = double(b1 * dtf);
+    C.c4 = double(0.5f * b2 * dtf * dtf);
+  } else {
+    C.dt = dtd;
+    C.c1 = 0.5 * (1.0 - o.beta2) * dtd * dtd;
+    C.c2 = (1.0 - o.beta1) * dtd;
+    C.c3 = o.beta1 * dtd;
+    C.c4 = 0.5 * o.beta2 * dtd * dtd;
+  }
   femgpu_dyn_stats S{};
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   if (expl) {
+    // Steps are captured in a HIP graph of G steps when at least one full
+    // graph fits the run; the remainder (or the whole run) is launched eagerly.
     const int G = o.steps_per_graph > 0 ? o.steps_per_graph : 64;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < G; ++k) {
-      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
-      femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
-      femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM, da,
-                                    dv, nullptr, C, ctx->stream);
-    }
-    HIPCHK(hipStreamEndCapture(ctx->stream, &g));
-    HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
     int done = 0;
-    while (done + G <= endnt) {
-      femgpu::launch_set_int(dstep, done + 1, ctx->stream);
-      HIPCHK(hipGraphLaunch(exec, ctx->stream));
-      done += G;
+    if (G <= endnt) {
+      hipGraphExec_t exec = nullptr;
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < G; ++k) {
+        femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
+        femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+        femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM,
+                                      da, dv, nullptr, C, ctx->stream);
+      }
+      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+      const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(ie);
+      while (done + G <= endnt) {
+        femgpu::launch_set_int(dstep, done + 1, ctx->stream);
+        const hipError_t le = hipGraphLaunch(exec, ctx->stream);
+        if (le != hipSuccess) {
+          (void)hipStreamSynchronize(ctx->stream);
+          (void)hipGraphExecDestroy(exec);
+          HIPCHK(le);
+        }
+        done += G;
+      }
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      (void)hipGraphExecDestroy(exec);
     }
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipGraphExecDestroy(exec);
     femgpu::launch_set_int(dstep, done + 1, ctx->stream);
     for (int k = 0; done < endnt; ++k, ++done) {
       femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);

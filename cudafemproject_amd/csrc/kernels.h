@@ -65,6 +65,9 @@ struct OpShift {
 };
 
 int lds_bytes(int dim, int elem_bytes, int pn, int emax);
+// Per-kernel launch attributes (large dynamic LDS); call once per device
+// before any launch or capture.
+hipError_t prepare_kernels();
 
 // y = K x (standalone: patch kernel + interface gather).
 template <typename T>

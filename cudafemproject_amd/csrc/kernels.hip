@@ -971,14 +971,14 @@ PatchArgs<T> patch_args(const DevMesh &dm) {
 }
 
 template <typename T, int DIM, int MODE, int PT>
+hipError_t patch_attr() {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ebe_patch<T, DIM, MODE, PT>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+template <typename T, int DIM, int MODE, int PT>
 void launch_patch_pt(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
   const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.emax);
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ebe_patch<T, DIM, MODE, PT>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
   const int grid = 8 * a.p8;
   hipLaunchKernelGGL((k_ebe_patch<T, DIM, MODE, PT>), dim3(grid), dim3(PT), L.total, s, a);
 }
@@ -992,6 +992,24 @@ void launch_patch(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t prepare_kernels() {
+  // dynamic LDS above 64 KiB must be allowed per kernel, once, outside any
+  // stream capture
+  hipError_t e = hipSuccess, r;
+#define FEMGPU_ATTR(T, D, M, P) \
+  if ((r = patch_attr<T, D, M, P>()) != hipSuccess) e = r;
+#define FEMGPU_ATTR_TD(T, D)                                                   \
+  FEMGPU_ATTR(T, D, 0, 256) FEMGPU_ATTR(T, D, 1, 256) FEMGPU_ATTR(T, D, 2, 256) \
+  FEMGPU_ATTR(T, D, 0, 512) FEMGPU_ATTR(T, D, 1, 512) FEMGPU_ATTR(T, D, 2, 512)
+  FEMGPU_ATTR_TD(double, 3)
+  FEMGPU_ATTR_TD(double, 2)
+  FEMGPU_ATTR_TD(float, 3)
+  FEMGPU_ATTR_TD(float, 2)
+#undef FEMGPU_ATTR_TD
+#undef FEMGPU_ATTR
+  return e;
+}
 
 int lds_bytes(int dim, int elem_bytes, int pn, int emax) {
   return lds_layout(dim, elem_bytes, pn, emax).total;

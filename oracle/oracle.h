@@ -71,6 +71,8 @@ typedef struct {
   orc_wavelet wavelet;
   int compat_scramble;  /* 1: repeat the reference's in-place permutation of
                            Mlocals / diagK by sort_by_key (cpu_fem_utils.cpp:437-445) */
+  double pcg_eps;       /* implicit inner PCG stopping constant; 0 -> 1e-4 as
+                           the reference (femfunc.cpp:153)                      */
 } orc_dyn_params;
 
 /* D matrix, row-major 3x3 (2D) or 6x6 (3D) (datakeeper.cpp:96-114). */
@@ -122,6 +124,7 @@ int orc_stress_f64(const orc_problem *pb, const double *u, double *sigma_out);
 
 /* Wavelet value exactly as ElementsData::updateWavelet (float). */
 float orc_wavelet_value(double t, const orc_wavelet *w);
+double orc_wavelet_value_f64(double t, const orc_wavelet *w);
 
 #ifdef __cplusplus
 }

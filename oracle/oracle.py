@@ -43,7 +43,7 @@ class OrcWavelet(C.Structure):
 class OrcDyn(C.Structure):
     _fields_ = [("rho", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
                 ("dt", C.c_double), ("endtime", C.c_double), ("wavelet", OrcWavelet),
-                ("compat_scramble", C.c_int)]
+                ("compat_scramble", C.c_int), ("pcg_eps", C.c_double)]
 
 
 _lib = None
@@ -172,7 +172,7 @@ WAVELETS = {"": 0, "none": 0, "ricker": 1, "berlage": 2}
 
 
 def dyn_solve(pb, rho, beta1, beta2, dt, endtime, wavelet=None, dtype=np.float32,
-              compat_scramble=True):
+              compat_scramble=True, pcg_eps=0.0):
     sfx, dtp, P = _sfx(dtype)
     pin = _Pinned(pb)
     d = OrcDyn()
@@ -183,6 +183,7 @@ def dyn_solve(pb, rho, beta1, beta2, dt, endtime, wavelet=None, dtype=np.float32
     d.wavelet.timeshift = w.get("timeshift", 0.0)
     d.wavelet.ampl = w.get("ampl", 1.0)
     d.compat_scramble = int(compat_scramble)
+    d.pcg_eps = pcg_eps
     u = np.zeros((pb.n_nodes, pb.dim), dtp)
     n = C.c_int(0)
     f = getattr(lib(), "orc_dyn_solve" + sfx)

@@ -26,3 +26,22 @@ float orc_wavelet_value(double td, const orc_wavelet *w) {
   }
   return 0.f;
 }
+
+/* The same wavelets evaluated in double (the FP64 restatement). */
+double orc_wavelet_value_f64(double t, const orc_wavelet *w) {
+  double ts = w->timeshift, freq = w->freq, ampl = w->ampl;
+  if (t < ts) return 0.0;
+  t = t - ts;
+  if (w->type == 1) {
+    double tmp = ORC_PI * freq * t - ORC_PI;
+    return ampl * (1.0 - 2.0 * tmp * tmp) * exp(-1.0 * tmp * tmp);
+  }
+  if (w->type == 2) {
+    double w0 = 2 * ORC_PI * freq;
+    double w1 = w0 / sqrt(3.0);
+    return ampl * w1 * w1 / 4.0 * exp(-1.0 * w1 * t) *
+           (sin(w0 * t) * (1.0 / (w1 * w1 * w1) + t / (w1 * w1) - t * t / w1) -
+            cos(w0 * t) * sqrt(3.0) * (t * t / w1 + t / (w1 * w1)));
+  }
+  return 0.0;
+}

@@ -363,9 +363,10 @@ static void FN(ebe_mv)(const FN(mesh) *m, const R *K, const R *p, R *u) {
   }
 }
 
-/* updateWavelet (fem_utils.cpp:53-82), evaluated in float. */
+/* updateWavelet (fem_utils.cpp:53-82), in float (as the reference) for the
+ * FP32 restatement and in double for the FP64 one. */
 static R FN(wavelet)(double t, const orc_wavelet *w) {
-  return (R)orc_wavelet_value(t, w);
+  return sizeof(R) == sizeof(float) ? (R)orc_wavelet_value(t, w) : (R)orc_wavelet_value_f64(t, w);
 }
 
 /* Element-local RHS state: pressures can be overwritten by the wavelet value
@@ -659,7 +660,8 @@ int FN(orc_dyn_solve)(const orc_problem *pb, const orc_dyn_params *dp, R *u_out,
       FN(gather)(&m, c.tmpg, x_dyn);
     } else {
       int it = 0;
-      st = FN(pcg)(&m, K, diagK, &c, (R)1e-4f, 100000, perm, &it);
+      const R peps = dp->pcg_eps > 0 ? (R)dp->pcg_eps : (R)1e-4f;
+      st = FN(pcg)(&m, K, diagK, &c, peps, 100000, perm, &it);
       if (st) goto done;
     }
     R c3 = beta1 * dt;
