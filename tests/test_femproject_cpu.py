@@ -1,0 +1,74 @@
+"""CPU tests of the FEMproject-compatible host layer (fem_main / run_fem.py):
+run-config keys and defaults (datakeeper.cpp:6-48), prepared-mesh paths,
+both stress.txt layouts."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+FEM_MAIN = os.path.join(ROOT, "cudafemproject_amd", "fem_main")
+
+
+def workspace(tmp_path, fixture="test_rect_pcg", dim=2, **cfg):
+    prep = tmp_path / "prepared_meshes" / f"{dim}D" / fixture
+    prep.mkdir(parents=True)
+    for f in ("nodes.txt", "elements.txt", "constraints.txt", "loads.txt", "stress.txt"):
+        shutil.copy(os.path.join(GOLDEN, fixture, f), prep / f)
+    c = {"working_path": str(tmp_path) + "/", "task": [fixture], "DIM": dim}
+    c.update(cfg)
+    p = tmp_path / "cfg.json"
+    p.write_text(json.dumps(c))
+    return str(p)
+
+
+def dry(cfg, task="test_rect_pcg"):
+    r = subprocess.run([FEM_MAIN, cfg, task, "--dry-run"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_static_defaults(tmp_path):
+    out = dry(workspace(tmp_path))
+    assert "Nodes count = 6" in out and "Elements count = 4" in out
+    assert "Boundary edges with b.c. count = 2" in out
+    assert "task_type static" in out and "E 2e+07 nu 0.3" in out     # the reader's defaults
+    assert out.strip().endswith("fem_results/2D/test_rect_pcg/results.vtk")
+
+
+def test_dynamic_keys_only_for_dynamic(tmp_path):
+    cfg = workspace(tmp_path, youngModulus=3e7, poissonRatio=0.2, task_type="dynamics",
+                    rho=1000, dt=2e-4)
+    out = dry(cfg)
+    # "dynamics" (as several shipped configs say) is not "dynamic" for the reader
+    assert "task_type dynamics" in out and "rho 2400 dt 0.001" in out
+    cfg = workspace(tmp_path / "b", task_type="dynamic", rho=1000, dt=2e-4, beta2=0.6,
+                    wavelet={"WaveletType": "ricker", "freq": 50})
+    out = dry(cfg)
+    assert "task_type dynamic" in out and "rho 1000 dt 0.0002" in out and "beta2 0.6" in out
+    assert "wavelet ricker" in out
+
+
+def test_bad_config_is_an_error(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text('{"DIM": 2, "task": [')
+    r = subprocess.run([FEM_MAIN, str(p), "x", "--dry-run"], capture_output=True, text=True)
+    assert r.returncode == 1 and "JSON parse error" in r.stderr
+
+
+def test_missing_mesh_is_an_error(tmp_path):
+    p = tmp_path / "c.json"
+    p.write_text(json.dumps({"DIM": 3, "working_path": str(tmp_path)}))
+    r = subprocess.run([FEM_MAIN, str(p), "nothere", "--dry-run"], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_run_fem_reports_missing_prepared_mesh(tmp_path):
+    p = tmp_path / "c.json"
+    p.write_text(json.dumps({"DIM": 2, "working_path": str(tmp_path) + "/", "task": ["nope"]}))
+    r = subprocess.run(["python", os.path.join(ROOT, "run_fem.py"), str(p)], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "prepared mesh for nope not found" in r.stdout

@@ -1,0 +1,51 @@
+"""GPU: the whole run_fem.py -> fem_main -> results.vtk flow on the golden
+fixtures, and the argv form run_fem.py of the reference builds."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import ROOT
+from helpers import fixture_displacements, fixture_problem, fixture_stress, read_vtk_field
+from test_femproject_cpu import workspace
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("fixture", ["test_rect_pcg", "9task_3"])
+def test_run_fem_static_fixture(tmp_path, fixture):
+    cfg = workspace(tmp_path, fixture, youngModulus=2e7, poissonRatio=0.25)
+    r = subprocess.run(["python", os.path.join(ROOT, "run_fem.py"), cfg], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vtk = tmp_path / "fem_results" / "2D" / fixture / "results.vtk"
+    u = read_vtk_field(str(vtk), "displacements").reshape(-1, 3)[:, :2]
+    ug = fixture_displacements(fixture)
+    for c in range(2):
+        assert np.linalg.norm(u[:, c] - ug[:, c]) < 1e-5     # run_tests.py:42,67-69
+    s = read_vtk_field(str(vtk), "stress").reshape(-1, 3, 3)
+    sg = fixture_stress(fixture)
+    got = np.stack([s[:, 0, 0], s[:, 1, 1], s[:, 0, 1]], 1)
+    assert np.abs(got - sg).max() / np.abs(sg).max() < 1e-4
+
+
+def test_fem_main_argv_form_dynamic(tmp_path):
+    """run_fem.py:80-84 argv (task dim proj prep res nu E verbose 1 rho da db dt
+    endtime beta1 beta2): explicit dynamics on the test_rect_pcg mesh vs the
+    FP64 oracle."""
+    cfg = workspace(tmp_path, "test_rect_pcg")
+    prep = tmp_path / "prepared_meshes" / "2D" / "test_rect_pcg"
+    res = tmp_path / "res"
+    res.mkdir()
+    exe = os.path.join(ROOT, "cudafemproject_amd", "fem_main")
+    r = subprocess.run([exe, "test_rect_pcg", "2", ROOT, str(prep), str(res), "0.25", "2e7", "0",
+                        "1", "2400", "0", "0", "1e-6", "2e-4", "0.5", "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u = read_vtk_field(str(res / "results.vtk"), "displacements").reshape(-1, 3)[:, :2]
+    pb = fixture_problem("test_rect_pcg")
+    _, uo, n = O.dyn_solve(pb, 2400.0, 0.5, 0.0, 1e-6, 2e-4, None, np.float64,
+                           compat_scramble=False)
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-7   # 9 printed digits
