@@ -17,6 +17,7 @@ single-threaded CPU PCG, timed on a bounded sample and extrapolated).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -75,8 +76,168 @@ def cpu_baseline(pb, n_fp64_iters, cpu_iters):
     }
 
 
+def lumped_mass(pb, rho):
+    """Physical lumped mass per node: rho V / 4 of every tet at each corner."""
+    X = pb.nodes[pb.elems]
+    V = np.abs(np.einsum("ei,ei->e", X[:, 1] - X[:, 0],
+                         np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))) / 6.0
+    m = np.zeros(pb.n_nodes)
+    np.add.at(m, pb.elems.ravel(), np.repeat(rho * V / 4.0, 4))
+    return m
+
+
+def omega_max(ctx, pb, m, iters=60):
+    """Largest eigenvalue of M^-1 K by power iteration (free DOFs), as omega."""
+    free = ~pb.fixed_dofs()
+    x = np.random.default_rng(0).standard_normal((pb.n_nodes, 3)) * free
+    lam = 0.0
+    for _ in range(iters):
+        y = ctx.matvec(x, 0) / m[:, None] * free
+        lam = float(np.vdot(x, y) / np.vdot(x, x))
+        x = y / np.linalg.norm(y)
+    return np.sqrt(lam)
+
+
+def bench_column(args, F):
+    """Explicit GN22 (central difference, lumped mass) on the 16x16x256 column,
+    Ricker 50 Hz top pressure, dt = 0.5 * 2/omega_max, 20,000 steps."""
+    from cudafemproject_amd.problem import column_problem
+    pb = column_problem(16, 16, 256, 0.1, 0.1, 1.6, E=2e7, nu=0.25, pressure=1.0,
+                        convention=F.LOADS_PHYSICAL)
+    rho = 1000.0
+    ctx = F.Context(0)
+    ctx.set_problem(pb)
+    w = omega_max(ctx, pb, lumped_mass(pb, rho))
+    dt = 0.5 * 2.0 / w
+    nsteps = args.dyn_steps
+    log(f"column: {pb.n_elems} elems, {pb.n_dof} dof, omega_max {w:.4e}, dt {dt:.4e}, "
+        f"{nsteps} steps")
+    res = {}
+    for name, prec in (("fp64", F.FP64), ("fp32", F.FP32)):
+        kw = dict(rho=rho, beta1=0.5, beta2=0.0, dt=dt, endtime=(nsteps + 0.5) * dt,
+                  wavelet_type=F.WAVELET_RICKER, freq=50.0, timeshift=0.0, ampl=1e3,
+                  precision=prec, mass_convention=F.MASS_PHYSICAL)
+        for _ in range(args.warmup):
+            ctx.dyn_run(**dict(kw, endtime=200 * dt))
+        runs = [ctx.dyn_run(**kw) for _ in range(args.steps)]
+        u, st = runs[-1]
+        ms = np.mean([r[1]["run_ms"] for r in runs])
+        res[name] = (ms, st["steps"], u)
+        log(f"{name}: {st['steps']} steps in {ms:.1f} ms = {ms * 1e3 / st['steps']:.2f} us/step, "
+            f"max |u| {np.abs(u).max():.3e}")
+    ms64, n64, u64 = res["fp64"]
+    ms32, n32, u32 = res["fp32"]
+    N = pb.n_dof
+    b64 = bytes_per_matvec(pb.n_elems, pb.n_nodes, N, 8) + 11 * 8 * N
+    b32 = bytes_per_matvec(pb.n_elems, pb.n_nodes, N, 4) + 11 * 4 * N
+    us64, us32 = ms64 * 1e3 / n64, ms32 * 1e3 / n32
+    ach = b64 / (us64 * 1e-6) / 1e9
+    line = {
+        "metric": "explicit GN22 step time (column 16x16x256, 20,000 steps)",
+        "value": us64, "unit": "us/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "column 0.1x0.1x1.6, 16x16x256 hexes x 6 tets, bottom clamped, "
+                               "Ricker 50 Hz x 1e3 top pressure, lumped mass rho=1000, "
+                               "dt = 0.5 * 2/omega_max", "n_elems": pb.n_elems, "n_dof": N,
+                   "dt": dt, "time_steps": n64},
+        "us_per_step_fp32": us32, "steps_per_s_fp64": 1e6 / us64, "steps_per_s_fp32": 1e6 / us32,
+        "fp32_vs_fp64_rel_displacement_diff": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one explicit step fp64 (predict + EbE matvec + correct)",
+                     "bytes_per_launch": b64},
+        "fp32_step_gbs": b32 / (us32 * 1e-6) / 1e9,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_cantilever(args, F):
+    """Compliance and its adjoint gradient on the 96x32x32 cantilever; central
+    FD (h = 1e-6 in rho_e) on 64 seeded elements.  The FD evaluates compliance
+    in its stationary form 2 f'u - u'Ku (the PCG error enters quadratically)
+    and sums the terms of C(+h) - C(-h) exactly (math.fsum), so it is limited
+    by the rounding of individual terms, not by the PCG tolerance or by the
+    rounding of C itself (a one-element change moves C by ~1e-11)."""
+    from cudafemproject_amd.problem import cantilever_problem
+    pb = cantilever_problem()
+    ctx = F.Context(0)
+    ctx.set_problem(pb)
+    rho = pb.rho
+    ctx.compliance_grad(F.FP64, tol=1e-3)
+    times = {}
+    for name, prec in (("fp64", F.FP64), ("mixed", F.MIXED)):
+        for _ in range(args.warmup):
+            ctx.compliance_grad(prec, tol=1e-8)
+        t = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            c, g, u, st = ctx.compliance_grad(prec, tol=1e-8)
+            t.append(time.perf_counter() - t0)
+        times[name] = (float(np.mean(t)), st, c, g)
+        log(f"{name}: C={c:.9e} {st['iterations']} it, solve {st['solve_ms']:.1f} ms, "
+            f"with gradient and host copies {np.mean(t) * 1e3:.1f} ms")
+    t64, s64, c64, g64 = times["fp64"]
+    tmx, smx, cmx, gmx = times["mixed"]
+
+    f = ctx.rhs().ravel()
+
+    def c_stat_terms(scale):
+        """Terms of the stationary compliance 2 f'u - u'Ku = sum(2 f u) + sum(s g)."""
+        ctx.set_material(pb.E, pb.nu, scale)
+        _, g, u, _ = ctx.compliance_grad(F.FP64, tol=1e-12)
+        return np.concatenate([2.0 * f * u.ravel(), scale * g]), g
+
+    _, g0 = c_stat_terms(rho ** 3)
+    adj = 3.0 * rho ** 2 * g0
+    h = 1e-6
+    sample = np.random.default_rng(0).choice(pb.n_elems, args.fd_elems, replace=False)
+    fd = []
+    for i, k in enumerate(sample):
+        if i % 16 == 0:
+            log(f"FD element {i}/{len(sample)}")
+        ts = []
+        for sgn in (1.0, -1.0):
+            r = rho.copy()
+            r[k] += sgn * h
+            ts.append(c_stat_terms(r ** 3)[0])
+        # exactly rounded sum of all terms of C(+h) - C(-h): no cancellation
+        # error beyond the rounding of the individual terms
+        fd.append(math.fsum(np.concatenate([ts[0], -ts[1]])) / (2 * h))
+    fd = np.array(fd)
+    err = np.abs(fd - adj[sample])
+    fd_rel_max = float(err.max() / np.abs(fd).max())
+    fd_rel_med = float(np.median(err / np.abs(fd)))
+    log(f"FD check on {len(sample)} elements: max|adj-fd|/max|fd| {fd_rel_max:.2e}, "
+        f"median per-element {fd_rel_med:.2e}")
+    line = {
+        "metric": "compliance + adjoint gradient time (cantilever 96x32x32)",
+        "value": tmx, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tmx * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "cantilever 3x1x1, 96x32x32 hexes x 6 tets, rho_e ~ U[0.05,1] "
+                               "seed 5, E_e = 2e7 rho_e^3, nu 0.3, x=0 clamped, tip pressure",
+                   "n_elems": pb.n_elems, "n_dof": pb.n_dof, "tol": 1e-8},
+        "fp64_s": t64, "iterations_fp64": s64["iterations"], "iterations_mixed": smx["iterations"],
+        "mixed_vs_fp64_grad_rel_l2": float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64)),
+        "fd_elements": int(len(sample)), "fd_h": h,
+        "fd_max_abs_err_over_max_fd": fd_rel_max, "fd_median_rel_err": fd_rel_med,
+        "homogeneity_rel": float(abs(np.dot(pb.elem_scale, g64) + c64) / abs(c64)),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["cube", "column", "cantilever"], default="cube",
+                    help="cube: static solve (default, headline); column: 20,000-step explicit "
+                         "dynamics; cantilever: compliance + adjoint gradient")
+    ap.add_argument("--dyn-steps", type=int, default=20000)
+    ap.add_argument("--fd-elems", type=int, default=64)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -92,6 +253,10 @@ def main():
 
     import cudafemproject_amd as F
     from cudafemproject_amd.problem import cube_problem
+    if args.workload == "column":
+        return bench_column(args, F)
+    if args.workload == "cantilever":
+        return bench_cantilever(args, F)
 
     t0 = time.perf_counter()
     pb = cube_problem(args.n, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)

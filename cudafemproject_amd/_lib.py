@@ -82,6 +82,9 @@ SIGNATURES = {
     "femgpu_solve_static": (C.c_int, [_P, C.POINTER(SolveOpts), _D, C.POINTER(Stats)]),
     "femgpu_stress": (C.c_int, [_P, _D, _D]),
     "femgpu_element_energy": (C.c_int, [_P, _D, _D]),
+    "femgpu_solve_rhs": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, C.POINTER(Stats)]),
+    "femgpu_material_grad": (C.c_int, [_P, _D, _D, _D]),
+    "femgpu_compliance_grad": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, _D, C.POINTER(Stats)]),
     "femgpu_default_dyn_opts": (C.c_int, [C.POINTER(DynOpts)]),
     "femgpu_dyn_run": (C.c_int, [_P, C.POINTER(DynOpts), _D, C.POINTER(DynStats)]),
 }
@@ -258,21 +261,62 @@ class Context:
         self._check(self.lib.femgpu_sync(self.h), "sync")
 
     # --- solves ----------------------------------------------------------------
-    def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
-              max_iter: int = 0, inner_tol: float = 0.1, check_every: int = 32,
-              raise_on_error: bool = True):
+    def _opts(self, precision, tol, tol_kind, max_iter, inner_tol, check_every) -> SolveOpts:
         o = SolveOpts()
         self.lib.femgpu_default_solve_opts(C.byref(o))
         o.precision, o.tol, o.tol_kind, o.max_iter = precision, tol, tol_kind, max_iter
         o.inner_tol, o.check_every = inner_tol, check_every
+        return o
+
+    @staticmethod
+    def _stats(s: Stats) -> dict:
+        return {"status": s.status, "iterations": s.iterations,
+                "outer_iterations": s.outer_iterations, "rel_res": s.rel_res,
+                "gamma_ratio": s.gamma_ratio, "solve_ms": s.solve_ms}
+
+    def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
+              max_iter: int = 0, inner_tol: float = 0.1, check_every: int = 32,
+              raise_on_error: bool = True):
+        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every)
         u = np.zeros((self.n_nodes, self.dim))
         s = Stats()
         st = self.lib.femgpu_solve_static(self.h, C.byref(o), _dp(u), C.byref(s))
         if raise_on_error:
             self._check(st, "solve_static")
-        return u, {"status": s.status, "iterations": s.iterations,
-                   "outer_iterations": s.outer_iterations, "rel_res": s.rel_res,
-                   "gamma_ratio": s.gamma_ratio, "solve_ms": s.solve_ms}
+        return u, self._stats(s)
+
+    def solve_rhs(self, f: np.ndarray, precision: int = FP64, tol: float = 1e-8,
+                  tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
+                  check_every: int = 32):
+        """Solve K u = f for an explicit right-hand side (n_nodes x dim)."""
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n_nodes, self.dim)
+        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every)
+        u = np.zeros((self.n_nodes, self.dim))
+        s = Stats()
+        self._check(self.lib.femgpu_solve_rhs(self.h, C.byref(o), _dp(f), _dp(u), C.byref(s)),
+                    "solve_rhs")
+        return u, self._stats(s)
+
+    def material_grad(self, u: np.ndarray, w: np.ndarray) -> np.ndarray:
+        """g_e = -w_e' K~_e^0 u_e: dL/ds_e for adjoint solution w."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        g = np.zeros(self.n_elems)
+        self._check(self.lib.femgpu_material_grad(self.h, _dp(u), _dp(w), _dp(g)), "material_grad")
+        return g
+
+    def compliance_grad(self, precision: int = FP64, tol: float = 1e-8,
+                        tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
+                        check_every: int = 32):
+        """C = f'u and dC/ds_e from one solve; returns (C, g, u, stats)."""
+        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every)
+        u = np.zeros((self.n_nodes, self.dim))
+        g = np.zeros(self.n_elems)
+        c = C.c_double(0)
+        s = Stats()
+        self._check(self.lib.femgpu_compliance_grad(self.h, C.byref(o), _dp(u), C.byref(c), _dp(g),
+                                                    C.byref(s)), "compliance_grad")
+        return c.value, g, u, self._stats(s)
 
     def stress(self, u: np.ndarray) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -912,10 +912,12 @@ int femgpu_default_solve_opts(femgpu_solve_opts *o) {
   return FEMGPU_OK;
 }
 
-int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
-                        femgpu_stats *stats) {
-  int st = need_mesh(ctx);
-  if (st) return st;
+// Static solve of K x = f with f the device vector `f_name` (n_nodes x dim,
+// internal order); the solution stays in the device buffer "sol64" and is
+// copied to u_out (user order) when u_out is given.
+static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char *f_name,
+                      double *u_out, femgpu_stats *stats) {
+  int st;
   femgpu_solve_opts o;
   femgpu_default_solve_opts(&o);
   if (opts) o = *opts;
@@ -929,13 +931,7 @@ int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *
   const size_t nd = size_t(ctx->nn) * ctx->dim;
   double *f64, *x64;
   CgScalars *sc, *sc2;
-  if (ctx->dirty_rhs) {
-    std::vector<double> fu;
-    host_rhs(ctx, false, fu);
-    if ((st = upload(ctx, "f64", to_internal(ctx, fu.data()), &f64))) return st;
-    ctx->dirty_rhs = false;
-  }
-  if ((st = dbuf(ctx, "f64", nd, &f64)) || (st = dbuf(ctx, "sol64", nd, &x64)) ||
+  if ((st = dbuf(ctx, f_name, nd, &f64)) || (st = dbuf(ctx, "sol64", nd, &x64)) ||
       (st = dbuf(ctx, "sc_main", 1, &sc)) || (st = dbuf(ctx, "sc_outer", 1, &sc2)))
     return st;
   femgpu_stats S{};
@@ -1054,6 +1050,89 @@ int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *
     ctx->err = "solve stopped with status " + std::to_string(S.status);
     return S.status;
   }
+  return FEMGPU_OK;
+}
+
+// The assembled load vector of the current problem on the device ("f64").
+static int ensure_rhs(femgpu_ctx *ctx) {
+  if (!ctx->dirty_rhs) return FEMGPU_OK;
+  std::vector<double> fu;
+  host_rhs(ctx, false, fu);
+  double *f64;
+  int st = upload(ctx, "f64", to_internal(ctx, fu.data()), &f64);
+  if (st) return st;
+  ctx->dirty_rhs = false;
+  return FEMGPU_OK;
+}
+
+int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
+                        femgpu_stats *stats) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if ((st = ensure_rhs(ctx))) return st;
+  return solve_core(ctx, opts, "f64", u_out, stats);
+}
+
+int femgpu_solve_rhs(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const double *f,
+                     double *u_out, femgpu_stats *stats) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!f) return fail(ctx, FEMGPU_E_ARG, "null right-hand side");
+  std::vector<double> fi = to_internal(ctx, f);
+  // rows of nodes no element references carry no equation
+  for (int64_t i = ctx->plan.n_owned + ctx->plan.n_iface; i < ctx->nn; ++i)
+    for (int c = 0; c < ctx->dim; ++c) fi[size_t(i) * ctx->dim + c] = 0.0;
+  double *df;
+  if ((st = upload(ctx, "f_user", fi, &df))) return st;
+  return solve_core(ctx, opts, "f_user", u_out, stats);
+}
+
+int femgpu_material_grad(femgpu_ctx *ctx, const double *u, const double *w, double *g_out) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!u || !w || !g_out) return fail(ctx, FEMGPU_E_ARG, "null argument");
+  if ((st = ensure_material(ctx)) || (st = ensure_fix(ctx))) return st;
+  std::vector<double> ui = to_internal(ctx, u), wi = to_internal(ctx, w), g(ctx->ne);
+  double *du, *dw, *dg;
+  if ((st = upload(ctx, "st_u", ui, &du)) || (st = upload(ctx, "gr_w", wi, &dw)) ||
+      (st = dbuf(ctx, "en_e", g.size(), &dg)))
+    return st;
+  femgpu::launch_bilinear<double>(ctx->dm, du, dw, -1.0, dg, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g.data(), dg, g.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < ctx->ne; ++i) g_out[ctx->plan.elem_perm[i]] = g[i];
+  return FEMGPU_OK;
+}
+
+int femgpu_compliance_grad(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
+                           double *c_out, double *g_out, femgpu_stats *stats) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!c_out || !g_out) return fail(ctx, FEMGPU_E_ARG, "null argument");
+  if ((st = ensure_rhs(ctx))) return st;
+  femgpu_stats S{};
+  st = solve_core(ctx, opts, "f64", u_out, &S);
+  if (stats) *stats = S;
+  if (st) return st;
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  double *f64, *x64, *dg;
+  if ((st = dbuf(ctx, "f64", nd, &f64)) || (st = dbuf(ctx, "sol64", nd, &x64)) ||
+      (st = dbuf(ctx, "en_e", size_t(ctx->ne), &dg)))
+    return st;
+  // K is symmetric and f does not depend on the element scales, so the
+  // adjoint of C = f'u is u itself: dC/ds_e = -u_e' K~_e^0 u_e.
+  femgpu::launch_bilinear<double>(ctx->dm, x64, x64, -1.0, dg, ctx->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> fh(nd), xh(nd), g(ctx->ne);
+  HIPCHK(hipMemcpyAsync(fh.data(), f64, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(xh.data(), x64, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(g.data(), dg, g.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double c = 0.0;
+  for (size_t i = 0; i < nd; ++i) c += fh[i] * xh[i];
+  *c_out = c;
+  for (int64_t i = 0; i < ctx->ne; ++i) g_out[ctx->plan.elem_perm[i]] = g[i];
   return FEMGPU_OK;
 }
 

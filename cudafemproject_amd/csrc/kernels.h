@@ -121,6 +121,8 @@ template <typename T>
 void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s);
 template <typename T>
 void launch_energy(const DevMesh &dm, const T *u, T *en, hipStream_t s);
+template <typename T>
+void launch_bilinear(const DevMesh &dm, const T *u, const T *w, double factor, T *out, hipStream_t s);
 
 // Dynamics.
 struct DynCoeffs {

@@ -866,6 +866,34 @@ __global__ __launch_bounds__(NT) void k_energy(int64_t ne, const int32_t *elems,
   en[e] = elem_force<T, DIM>(X, U, lam, mu, es, fix, F);
 }
 
+// out_e = factor * w_e' K~_e^0 u_e: the element stiffness at unit modulus
+// scale with the constrained-DOF treatment of K (off-diagonal couplings of a
+// constrained DOF dropped, its diagonal kept).  K = sum_e s_e K~_e^0, so with
+// factor -1 and w the adjoint solution this is dL/ds_e.
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_bilinear(int64_t ne, const int32_t *elems, const T *coords,
+                                                 const uint8_t *fixmask, T lam, T mu, const T *u,
+                                                 const T *w, T factor, T *out) {
+  constexpr int NPE = DIM + 1, NDE = DIM * NPE;
+  const int64_t e = int64_t(blockIdx.x) * NT + threadIdx.x;
+  if (e >= ne) return;
+  T X[NDE], U[NDE], W[NDE], F[NDE];
+  load_elem<T, DIM>(elems, coords, u, e, X, U);
+  unsigned fix = 0;
+#pragma unroll
+  for (int n = 0; n < NPE; ++n) {
+    const int64_t g = elems[e * NPE + n];
+    fix |= unsigned(fixmask[g] & ((1 << DIM) - 1)) << (n * DIM);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) W[n * DIM + c] = w[g * DIM + c];
+  }
+  (void)elem_force<T, DIM>(X, U, lam, mu, T(1), fix, F);
+  T acc = T(0);
+#pragma unroll
+  for (int i = 0; i < NDE; ++i) acc += W[i] * F[i];
+  out[e] = factor * acc;
+}
+
 // ---------------------------------------------------------------------------
 // Dynamics (GN22, femfunc.cpp:247-343) on assembled vectors.
 
@@ -1202,6 +1230,20 @@ void launch_energy(const DevMesh &dm, const T *u, T *en, hipStream_t s) {
 }
 
 template <typename T>
+void launch_bilinear(const DevMesh &dm, const T *u, const T *w, double factor, T *out,
+                     hipStream_t s) {
+  const T *coords = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.coords64)
+                                   : reinterpret_cast<const T *>(dm.coords32);
+  const int g = int((dm.ne + NT - 1) / NT);
+  if (dm.dim == 3)
+    hipLaunchKernelGGL((k_bilinear<T, 3>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords,
+                       dm.fixmask, T(dm.lam), T(dm.mu), u, w, T(factor), out);
+  else
+    hipLaunchKernelGGL((k_bilinear<T, 2>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords,
+                       dm.fixmask, T(dm.lam), T(dm.mu), u, w, T(factor), out);
+}
+
+template <typename T>
 void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
                         hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
@@ -1256,6 +1298,7 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_copy<T>(const T *, T *, int64_t, hipStream_t);                          \
   template void launch_stress<T>(const DevMesh &, const T *, T *, hipStream_t);                \
   template void launch_energy<T>(const DevMesh &, const T *, T *, hipStream_t);                \
+  template void launch_bilinear<T>(const DevMesh &, const T *, const T *, double, T *, hipStream_t); \
   template void launch_dyn_predict<T>(const DevMesh &, T *, T *, T *, DynCoeffs, bool,         \
                                       hipStream_t);                                            \
   template void launch_dyn_correct<T>(const DevMesh &, const T *, const T *, const T *,        \

@@ -174,6 +174,28 @@ int femgpu_default_solve_opts(femgpu_solve_opts *opts);
 int femgpu_solve_static(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
                         femgpu_stats *stats);
 
+/* Solve K u = f for an explicit right-hand side f (n_nodes x dim, user node
+ * order) instead of the problem's loads; the same solver and options as
+ * femgpu_solve_static.  Replaces: gpuPCG_EbE_vec called on a caller-filled
+ * Flocals (femfunc.cpp:25-128), which the reference only fills from its
+ * loads; here it is also the adjoint solve (K is symmetric). */
+int femgpu_solve_rhs(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const double *f,
+                     double *u_out, femgpu_stats *stats);
+
+/* --- adjoint material gradient ----------------------------------------------
+ * No counterpart in the reference (it has no differentiation).  With element
+ * modulus scales s_e (femgpu_set_material's elem_scale), K = sum_e s_e K~_e^0
+ * where K~_e^0 is the element stiffness at scale 1 with the constrained-DOF
+ * treatment of K.  For a loss L(u) with K u = f and adjoint K w = dL/du:
+ *   dL/ds_e = -w_e' K~_e^0 u_e   (written to g_out, n_elems, user order).
+ * K~_e^0 is recomputed on the fly, never stored. */
+int femgpu_material_grad(femgpu_ctx *ctx, const double *u, const double *w, double *g_out);
+/* Compliance C = f'u of the problem's loads and dC/ds_e = -u_e' K~_e^0 u_e
+ * from ONE forward solve (the adjoint solution of C is u itself).
+ * u_out (n_nodes x dim) may be NULL; c_out: 1 value; g_out: n_elems. */
+int femgpu_compliance_grad(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
+                           double *c_out, double *g_out, femgpu_stats *stats);
+
 /* --- per-element results ----------------------------------------------------
  * No counterpart in the reference's live code; the golden fixtures hold
  * sigma = D B u_e per element (testing/<task>_test/results.vtk, TENSORS stress).

@@ -54,3 +54,20 @@ def dense_K(pb):
         e[j] = 1.0
         K[:, j] = O.matvec(pb, e.reshape(pb.n_nodes, pb.dim)).ravel()
     return K
+
+
+def elem_dofs(pb):
+    """(n_elems, dim*(dim+1)) global DOF ids, DOF = dim*node + d."""
+    d = pb.dim
+    return (pb.elems[:, :, None].astype(np.int64) * d + np.arange(d)).reshape(pb.n_elems, -1)
+
+
+def material_grad_oracle(pb, u, w):
+    """-w_e' K~_e^0 u_e with K~_e^0 the oracle's constrained element stiffness
+    at unit modulus scale (klocals of the problem without elem_scale)."""
+    import dataclasses
+    import oracle as O
+    K0, _ = O.klocals(dataclasses.replace(pb, elem_scale=None), np.float64)
+    dofs = elem_dofs(pb)
+    ue, we = np.asarray(u).ravel()[dofs], np.asarray(w).ravel()[dofs]
+    return -np.einsum("ei,eij,ej->e", we, K0, ue)

@@ -1,0 +1,138 @@
+"""Adjoint material gradient and the torch.autograd binding on the GPU.
+
+K(s) = sum_e s_e K~_e^0 with s_e the element modulus scale; for compliance
+C = f'u, dC/ds_e = -u_e' K~_e^0 u_e.  Checks: the gradient kernel against the
+oracle's element stiffness, central finite differences of C (h = 1e-6 in the
+density rho_e, s_e = rho_e^3), the homogeneity identity sum_e s_e dC/ds_e = -C
+at the full cantilever size, and torch.autograd.gradcheck of the binding."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cudafemproject_amd as F
+from cudafemproject_amd.autograd import ElasticProblem
+from cudafemproject_amd.problem import cantilever_problem, plate_problem
+from helpers import dense_K, material_grad_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("kind", ["tet", "tri"])
+def test_material_grad_vs_oracle(ctx, kind):
+    if kind == "tet":
+        pb = cantilever_problem(6, 3, 2, lx=2.0, seed=3)
+    else:
+        pb = plate_problem(9, 5, E=2e7, nu=0.25)
+    print(f"[material_grad] {kind} n_elems={pb.n_elems}", flush=True)
+    ctx.set_problem(pb)
+    rng = np.random.default_rng(7)
+    u, w = rng.standard_normal((2, pb.n_nodes, pb.dim))
+    g = ctx.material_grad(u, w)
+    assert rel(g, material_grad_oracle(pb, u, w)) < 1e-12     # FP64, same formula order aside
+
+
+def test_solve_rhs_vs_dense(ctx):
+    pb = cantilever_problem(5, 2, 2, lx=2.0, seed=9)
+    ctx.set_problem(pb)
+    f = np.random.default_rng(1).standard_normal((pb.n_nodes, pb.dim))
+    f[pb.fixed_dofs()] = 0.0
+    u, st = ctx.solve_rhs(f, F.FP64, tol=1e-13)
+    ud = np.linalg.solve(dense_K(pb), f.ravel())
+    assert st["status"] == 0
+    assert rel(u, ud) < 1e-9
+
+
+def _c_stat_terms(scale, ctx, pb, f):
+    """Terms of compliance in its stationary form 2 f'u - u'Ku = sum(2 f u) +
+    sum(s g): the solver's error enters quadratically, so a finite difference
+    of it is limited by rounding, not by the PCG tolerance."""
+    ctx.set_material(pb.E, pb.nu, scale)
+    _, g, u, st = ctx.compliance_grad(F.FP64, tol=1e-12)
+    assert st["status"] == 0
+    return np.concatenate([2.0 * f * u.ravel(), scale * g]), g
+
+
+def test_compliance_grad_central_fd(ctx):
+    pb = cantilever_problem(12, 4, 4, lx=3.0, seed=5)
+    print(f"[compliance fd] n_elems={pb.n_elems}", flush=True)
+    ctx.set_problem(pb)
+    f = ctx.rhs().ravel()
+    rho = pb.rho
+    _, g = _c_stat_terms(rho ** 3, ctx, pb, f)
+    dC_drho = 3.0 * rho ** 2 * g
+    h = 1e-6
+    sample = np.random.default_rng(0).choice(pb.n_elems, 64, replace=False)
+    fd = np.empty(len(sample))
+    for i, k in enumerate(sample):
+        ts = []
+        for sgn in (1.0, -1.0):
+            r = rho.copy()
+            r[k] += sgn * h
+            ts.append(_c_stat_terms(r ** 3, ctx, pb, f)[0])
+        fd[i] = math.fsum(np.concatenate([ts[0], -ts[1]])) / (2 * h)
+    err = np.abs(fd - dC_drho[sample])
+    print(f"  max |adj-fd| / max|fd| = {err.max() / np.abs(fd).max():.3e}, "
+          f"max per-element {np.max(err / np.abs(fd)):.3e}", flush=True)
+    assert err.max() <= 1e-5 * np.abs(fd).max()
+    assert np.all(err <= 1e-5 * np.abs(fd))
+
+
+def test_compliance_grad_full_cantilever(ctx):
+    """96x32x32 cantilever (589,824 tets): homogeneity sum_e s_e dC/ds_e = -C,
+    mixed-precision gradient against the FP64 gradient."""
+    pb = cantilever_problem()
+    print(f"[compliance full] n_elems={pb.n_elems}", flush=True)
+    ctx.set_problem(pb)
+    c, g, _, st = ctx.compliance_grad(F.FP64, tol=1e-10)
+    print(f"  FP64: C={c:.9e} it={st['iterations']} {st['solve_ms']:.1f} ms", flush=True)
+    assert st["status"] == 0
+    s = pb.elem_scale
+    assert abs(np.dot(s, g) + c) <= 1e-8 * abs(c)
+    cm, gm, _, stm = ctx.compliance_grad(F.MIXED, tol=1e-10)
+    print(f"  MIXED: it={stm['iterations']} {stm['solve_ms']:.1f} ms rel={rel(gm, g):.2e}", flush=True)
+    assert abs(cm - c) <= 1e-8 * abs(c)
+    assert rel(gm, g) < 1e-7
+
+
+def test_autograd_gradcheck(ctx):
+    pb = cantilever_problem(3, 2, 2, lx=1.5, seed=2)
+    pb.face_adj = pb.face_adj[:0]          # loads come from the f input
+    ctx.set_problem(pb)
+    ep = ElasticProblem(ctx, pb.E, pb.nu, tol=1e-13)
+    free = ~pb.fixed_dofs()
+    free_t = torch.from_numpy(free)
+    rng = np.random.default_rng(4)
+    s0 = torch.tensor(rng.uniform(0.5, 1.0, pb.n_elems), dtype=torch.float64, requires_grad=True)
+    f0 = torch.tensor(rng.standard_normal((pb.n_nodes, pb.dim)) * free * 1e3,
+                      dtype=torch.float64, requires_grad=True)
+
+    def fn(s, f):
+        return ep.solve(s, f * free_t) * 1e4
+
+    assert torch.autograd.gradcheck(fn, (s0, f0), eps=1e-6, atol=1e-6, rtol=1e-5)
+
+
+def test_autograd_compliance_matches_kernel(ctx):
+    pb = cantilever_problem(8, 3, 3, lx=2.0, seed=6)
+    ctx.set_problem(pb)
+    ep = ElasticProblem(ctx, pb.E, pb.nu, tol=1e-11)
+    rho = torch.tensor(pb.rho, dtype=torch.float64, requires_grad=True)
+    C = ep.compliance(rho ** 3)
+    C.backward()
+    ctx.set_material(pb.E, pb.nu, pb.rho ** 3)
+    c, g, _, _ = ctx.compliance_grad(F.FP64, tol=1e-11)
+    assert abs(C.item() - c) <= 1e-12 * abs(c)
+    assert rel(rho.grad.numpy(), 3 * pb.rho ** 2 * g) < 1e-12
+    # the general path (ElasticSolve + autograd of f'u) gives the same gradient
+    f = torch.tensor(ctx.rhs(), dtype=torch.float64)
+    rho2 = rho.detach().clone().requires_grad_(True)
+    u = ep.solve(rho2 ** 3, f)
+    (f * u).sum().backward()
+    assert rel(rho2.grad.numpy(), rho.grad.numpy()) < 1e-8
