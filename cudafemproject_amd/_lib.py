@@ -54,9 +54,15 @@ class DynStats(C.Structure):
                 ("run_ms", C.c_double)]
 
 
+class BatchStat(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("replacements", C.c_int),
+                ("pad", C.c_int), ("rel_res", C.c_double)]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
+_L = C.POINTER(C.c_int64)
 
 # name -> (restype, argtypes); the list of exported symbols of include/femgpu.h
 SIGNATURES = {
@@ -82,6 +88,11 @@ SIGNATURES = {
     "femgpu_solve_static": (C.c_int, [_P, C.POINTER(SolveOpts), _D, C.POINTER(Stats)]),
     "femgpu_stress": (C.c_int, [_P, _D, _D]),
     "femgpu_element_energy": (C.c_int, [_P, _D, _D]),
+    "femgpu_batch_set_mesh": (C.c_int, [_P, C.c_int, C.c_int, _L, _D, _L, _I]),
+    "femgpu_batch_set_material": (C.c_int, [_P, _D, _D]),
+    "femgpu_batch_set_constraints": (C.c_int, [_P, _L, _I, _I]),
+    "femgpu_batch_solve": (C.c_int, [_P, C.c_int, _D, C.POINTER(SolveOpts), _D,
+                                     C.POINTER(BatchStat), _D]),
     "femgpu_solve_rhs": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, C.POINTER(Stats)]),
     "femgpu_material_grad": (C.c_int, [_P, _D, _D, _D]),
     "femgpu_compliance_grad": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, _D, C.POINTER(Stats)]),
@@ -317,6 +328,59 @@ class Context:
         self._check(self.lib.femgpu_compliance_grad(self.h, C.byref(o), _dp(u), C.byref(c), _dp(g),
                                                     C.byref(s)), "compliance_grad")
         return c.value, g, u, self._stats(s)
+
+    # --- batched solves ---------------------------------------------------------
+    def batch_set_mesh(self, meshes):
+        """meshes: list of (nodes (n_i, dim), elems (m_i, dim+1) instance-local ids)."""
+        dim = int(np.asarray(meshes[0][0]).shape[1])
+        noff = np.zeros(len(meshes) + 1, np.int64)
+        eoff = np.zeros(len(meshes) + 1, np.int64)
+        for i, (nd, el) in enumerate(meshes):
+            noff[i + 1] = noff[i] + len(nd)
+            eoff[i + 1] = eoff[i] + len(el)
+        nodes = np.ascontiguousarray(np.concatenate([np.asarray(m[0], np.float64) for m in meshes]))
+        elems = np.ascontiguousarray(np.concatenate([np.asarray(m[1], np.int32) for m in meshes]))
+        self._check(self.lib.femgpu_batch_set_mesh(self.h, dim, len(meshes), noff.ctypes.data_as(_L),
+                                                   _dp(nodes), eoff.ctypes.data_as(_L), _ip(elems)),
+                    "batch_set_mesh")
+        self._batch = (dim, noff)
+
+    def batch_set_material(self, E, nu):
+        E = np.ascontiguousarray(E, np.float64)
+        nu = np.ascontiguousarray(nu, np.float64)
+        self._check(self.lib.femgpu_batch_set_material(self.h, _dp(E), _dp(nu)), "batch_set_material")
+
+    def batch_set_constraints(self, cons):
+        """cons: list of (nodes, codes) per instance (instance-local node ids)."""
+        off = np.zeros(len(cons) + 1, np.int64)
+        for i, (n, _) in enumerate(cons):
+            off[i + 1] = off[i] + len(n)
+        nodes = np.ascontiguousarray(np.concatenate([np.asarray(c[0], np.int32) for c in cons]))
+        codes = np.ascontiguousarray(np.concatenate([np.asarray(c[1], np.int32) for c in cons]))
+        self._check(self.lib.femgpu_batch_set_constraints(self.h, off.ctypes.data_as(_L), _ip(nodes),
+                                                          _ip(codes)), "batch_set_constraints")
+
+    def batch_solve(self, rhs, precision: int = MIXED, tol: float = 1e-8, max_iter: int = 0,
+                    inner_tol: float = 0.1, raise_on_error: bool = True):
+        """rhs: list of (n_i, n_rhs, dim) arrays.  Returns (list of u arrays of
+        the same shapes, stats as a structured array (n_inst, n_rhs), device ms)."""
+        dim, noff = self._batch
+        nr = int(np.asarray(rhs[0]).shape[1])
+        f = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, nr, dim)
+                                                 for r in rhs]))
+        assert f.shape[0] == noff[-1]
+        u = np.zeros_like(f)
+        st = (BatchStat * (len(rhs) * nr))()
+        o = self._opts(precision, tol, TOL_REL_RES, max_iter, inner_tol, 32)
+        ms = C.c_double(0)
+        code = self.lib.femgpu_batch_solve(self.h, nr, _dp(f), C.byref(o), _dp(u), st, C.byref(ms))
+        if raise_on_error:
+            self._check(code, "batch_solve")
+        stats = np.array([(x.status, x.iterations, x.replacements, x.rel_res) for x in st],
+                         dtype=[("status", "i4"), ("iterations", "i4"), ("replacements", "i4"),
+                                ("rel_res", "f8")]).reshape(len(rhs), nr)
+        us = [u[noff[i]:noff[i + 1]] for i in range(len(rhs))]
+        return us, stats, ms.value
 
     def stress(self, u: np.ndarray) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function"]
 
-LIB_SOURCES = ["kernels.hip", "femgpu_api.cpp", "mesh_plan.cpp"]
+LIB_SOURCES = ["kernels.hip", "batch.hip", "femgpu_api.cpp", "batch.cpp", "mesh_plan.cpp"]
 
 
 def _hipcc() -> str:
@@ -42,13 +42,13 @@ def _run(cmd):
 
 def build_library(out: str = LIB) -> str:
     """Compile libfemgpu.so (always recompiles)."""
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = _hipcc()
-    objs = []
-    for src in LIB_SOURCES:
-        o = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        _run([hipcc, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
-              os.path.join(CSRC, src), "-o", o])
-        objs.append(o)
+    objs = [os.path.join(CSRC, src.replace(".", "_") + ".o") for src in LIB_SOURCES]
+    cmds = [[hipcc, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, src), "-o", o]
+            for src, o in zip(LIB_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=min(len(cmds), 4)) as ex:
+        list(ex.map(_run, cmds))
     tmp = out + ".tmp"
     _run([hipcc, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, out)

@@ -1,0 +1,412 @@
+// batch.cpp -- host side of the batched solve (femgpu_batch_* in femgpu.h):
+// plans every instance, concatenates the plans into one device layout
+// (batch.h) and launches the persistent PCG kernel (batch.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+#include "femgpu.h"
+
+namespace femgpu {
+
+struct BatchState {
+  int dim = 0, n_inst = 0;
+  int64_t total_nodes = 0, total_elems = 0;
+  std::vector<int64_t> node_off, elem_off;  // user offsets, n_inst + 1
+  std::vector<BInst> inst;
+  std::vector<int32_t> node_perm;  // internal global node -> user global node
+  std::vector<int32_t> node_inv;   // user global node -> internal global node
+  std::vector<int32_t> order;
+  std::vector<uint8_t> fix;        // per internal global node
+  int pn = 0, emax = 0;
+  int64_t n_slots = 0;
+  std::map<std::string, std::pair<void *, size_t>> bufs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+void batch_free(BatchState *b) {
+  if (!b) return;
+  for (auto &kv : b->bufs)
+    if (kv.second.first) (void)hipFree(kv.second.first);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  delete b;
+}
+
+namespace {
+
+// Named device buffer of at least n elements (reallocated when too small).
+template <typename T>
+hipError_t bbuf(BatchState *b, const std::string &name, size_t n, T **out) {
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  auto it = b->bufs.find(name);
+  if (it != b->bufs.end() && it->second.second >= bytes) {
+    *out = static_cast<T *>(it->second.first);
+    return hipSuccess;
+  }
+  if (it != b->bufs.end()) {
+    (void)hipFree(it->second.first);
+    b->bufs.erase(it);
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return e;
+  b->bufs[name] = {p, bytes};
+  *out = static_cast<T *>(p);
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t bup(BatchState *b, hipStream_t s, const std::string &name, const std::vector<T> &v,
+               T **out) {
+  hipError_t e = bbuf(b, name, v.size(), out);
+  if (e != hipSuccess || v.empty()) return e;
+  e = hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+double elem_det(int dim, const double *nodes, const int32_t *en) {
+  if (dim == 2) {
+    const double *a = &nodes[size_t(en[0]) * 2], *b = &nodes[size_t(en[1]) * 2],
+                 *c = &nodes[size_t(en[2]) * 2];
+    return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
+  }
+  const double *a = &nodes[size_t(en[0]) * 3], *b = &nodes[size_t(en[1]) * 3],
+               *c = &nodes[size_t(en[2]) * 3], *d = &nodes[size_t(en[3]) * 3];
+  const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double e3[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+  return e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+         e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+}
+
+}  // namespace
+}  // namespace femgpu
+
+using femgpu::BatchState;
+using femgpu::BInst;
+using femgpu::BStat;
+
+#define BCHK(call)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return femgpu::ctx_fail(ctx, FEMGPU_E_HIP,                                            \
+                              (std::string(#call) + ": " + hipGetErrorString(e_)).c_str()); \
+  } while (0)
+
+extern "C" {
+
+int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *node_off,
+                          const double *nodes, const int64_t *elem_off, const int32_t *elems) {
+  if (!ctx) return FEMGPU_E_ARG;
+  if ((dim != 2 && dim != 3) || n_inst < 1 || !node_off || !nodes || !elem_off || !elems)
+    return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "bad batch mesh arguments");
+  BCHK(hipSetDevice(femgpu::ctx_device(ctx)));
+  hipStream_t s = femgpu::ctx_stream(ctx);
+  BCHK(hipStreamSynchronize(s));
+  BatchState *&bp = femgpu::ctx_batch(ctx);
+  femgpu::batch_free(bp);
+  bp = nullptr;
+  auto *b = new BatchState();
+  const int npe = dim + 1;
+  const int emax_b = dim == 2 ? 256 : 128;
+  if (node_off[0] != 0 || elem_off[0] != 0) {
+    delete b;
+    return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "batch offsets must start at 0");
+  }
+  for (int i = 0; i < n_inst; ++i)
+    if (node_off[i + 1] < node_off[i] + npe || elem_off[i + 1] <= elem_off[i]) {
+      delete b;
+      return femgpu::ctx_fail(ctx, FEMGPU_E_BAD_MESH,
+                              ("instance " + std::to_string(i) + " has too few nodes or no element").c_str());
+    }
+  if (node_off[n_inst] >= (int64_t(1) << 31) / 16) {
+    delete b;
+    return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "batch too large for 32-bit node ids");
+  }
+  b->dim = dim;
+  b->n_inst = n_inst;
+  b->node_off.assign(node_off, node_off + n_inst + 1);
+  b->elem_off.assign(elem_off, elem_off + n_inst + 1);
+  b->total_nodes = node_off[n_inst];
+  b->total_elems = elem_off[n_inst];
+  b->node_perm.assign(b->total_nodes, 0);
+  b->node_inv.assign(b->total_nodes, 0);
+  b->fix.assign(b->total_nodes, 0);
+  std::vector<femgpu::PatchInfo> patches;
+  std::vector<uint32_t> lid;
+  std::vector<uint16_t> epos, nend;
+  std::vector<int32_t> shg, shs, ss;
+  std::vector<double> c64(size_t(b->total_nodes) * dim);
+  int64_t e_int = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    const int64_t nn = node_off[i + 1] - node_off[i], ne = elem_off[i + 1] - elem_off[i];
+    const double *nd = nodes + size_t(node_off[i]) * dim;
+    const int32_t *el = elems + size_t(elem_off[i]) * npe;
+    for (int64_t e = 0; e < ne; ++e) {
+      for (int a = 0; a < npe; ++a)
+        if (el[e * npe + a] < 0 || el[e * npe + a] >= nn) {
+          delete b;
+          return femgpu::ctx_fail(ctx, FEMGPU_E_BAD_MESH,
+                                  ("instance " + std::to_string(i) + ": node id out of range").c_str());
+        }
+      const double det = femgpu::elem_det(dim, nd, &el[e * npe]);
+      if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
+        delete b;
+        return femgpu::ctx_fail(ctx, FEMGPU_E_BAD_MESH,
+                                ("instance " + std::to_string(i) + ": degenerate element " +
+                                 std::to_string(e)).c_str());
+      }
+    }
+    femgpu::MeshPlan P;
+    const std::string msg = femgpu::build_plan(dim, nn, nd, ne, el, emax_b, P);
+    if (!msg.empty()) {
+      delete b;
+      return femgpu::ctx_fail(ctx, FEMGPU_E_BAD_MESH, ("instance " + std::to_string(i) + ": " + msg).c_str());
+    }
+    const int32_t N0 = int32_t(node_off[i]);
+    const int32_t P0 = int32_t(patches.size());
+    const int32_t SH0 = int32_t(shg.size()), PN0 = int32_t(nend.size());
+    const int32_t S0 = int32_t(b->n_slots), IF0 = int32_t(ss.size());
+    for (femgpu::PatchInfo pi : P.patches) {
+      pi.elem_start += int32_t(e_int);
+      pi.own_start += N0;
+      pi.sh_start += SH0;
+      pi.pnode_start += PN0;
+      patches.push_back(pi);
+    }
+    lid.insert(lid.end(), P.elem_lid.begin(), P.elem_lid.end());
+    epos.insert(epos.end(), P.epos.begin(), P.epos.end());
+    nend.insert(nend.end(), P.node_end.begin(), P.node_end.end());
+    for (int32_t g : P.sh_gid) shg.push_back(g + N0);
+    for (int32_t sl : P.sh_slot) shs.push_back(sl + S0);
+    for (int32_t st : P.slot_start) ss.push_back(st + S0);
+    for (int64_t j = 0; j < nn; ++j) {
+      b->node_perm[N0 + j] = N0 + P.node_perm[j];
+      b->node_inv[N0 + P.node_perm[j]] = int32_t(N0 + j);
+      for (int c = 0; c < dim; ++c)
+        c64[size_t(N0 + j) * dim + c] = nd[size_t(P.node_perm[j]) * dim + c];
+    }
+    BInst I{};
+    I.patch_begin = P0;
+    I.patch_end = int32_t(patches.size());
+    I.node_off = N0;
+    I.n_nodes = int32_t(nn);
+    I.n_owned = int32_t(P.n_owned);
+    I.n_active = int32_t(P.n_owned + P.n_iface);
+    I.iface_off = IF0;
+    I.slot_off = S0;
+    const double E = 2e7, nu = 0.3;
+    I.mu = E / (2.0 * (1.0 + nu));
+    I.lam = dim == 3 ? E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)) : E * nu / (1.0 - nu * nu);
+    b->inst.push_back(I);
+    b->pn = std::max(b->pn, P.pn_used);
+    b->emax = std::max(b->emax, P.emax_used);
+    b->n_slots += P.n_slots;
+    e_int += ne;
+  }
+  // largest instances first
+  b->order.resize(n_inst);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
+    return b->elem_off[a + 1] - b->elem_off[a] > b->elem_off[c + 1] - b->elem_off[c];
+  });
+  std::vector<float> c32(c64.begin(), c64.end());
+  femgpu::PatchInfo *dp;
+  uint32_t *dl;
+  uint16_t *de, *dn;
+  int32_t *dg, *dsl, *dss, *dor;
+  double *dc64;
+  float *dc32;
+  BInst *di;
+  uint8_t *df;
+  BCHK(femgpu::bup(b, s, "patches", patches, &dp));
+  BCHK(femgpu::bup(b, s, "lid", lid, &dl));
+  BCHK(femgpu::bup(b, s, "epos", epos, &de));
+  BCHK(femgpu::bup(b, s, "nend", nend, &dn));
+  BCHK(femgpu::bup(b, s, "shg", shg, &dg));
+  BCHK(femgpu::bup(b, s, "shs", shs, &dsl));
+  BCHK(femgpu::bup(b, s, "ss", ss, &dss));
+  BCHK(femgpu::bup(b, s, "order", b->order, &dor));
+  BCHK(femgpu::bup(b, s, "c64", c64, &dc64));
+  BCHK(femgpu::bup(b, s, "c32", c32, &dc32));
+  BCHK(femgpu::bup(b, s, "inst", b->inst, &di));
+  BCHK(femgpu::bup(b, s, "fix", b->fix, &df));
+  BCHK(hipEventCreate(&b->ev0));
+  BCHK(hipEventCreate(&b->ev1));
+  bp = b;
+  return FEMGPU_OK;
+}
+
+int femgpu_batch_set_material(femgpu_ctx *ctx, const double *E, const double *nu) {
+  if (!ctx) return FEMGPU_E_ARG;
+  BatchState *b = femgpu::ctx_batch(ctx);
+  if (!b) return femgpu::ctx_fail(ctx, FEMGPU_E_STATE, "no batch mesh");
+  if (!E || !nu) return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "null material arrays");
+  for (int i = 0; i < b->n_inst; ++i) {
+    if (!(E[i] > 0.0) || !(nu[i] > -1.0) || !(nu[i] < (b->dim == 3 ? 0.5 : 1.0)))
+      return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, ("bad material of instance " + std::to_string(i)).c_str());
+    BInst &I = b->inst[i];
+    I.mu = E[i] / (2.0 * (1.0 + nu[i]));
+    I.lam = b->dim == 3 ? E[i] * nu[i] / ((1.0 + nu[i]) * (1.0 - 2.0 * nu[i]))
+                        : E[i] * nu[i] / (1.0 - nu[i] * nu[i]);
+  }
+  BInst *di;
+  BCHK(femgpu::bup(b, femgpu::ctx_stream(ctx), "inst", b->inst, &di));
+  return FEMGPU_OK;
+}
+
+int femgpu_batch_set_constraints(femgpu_ctx *ctx, const int64_t *con_off, const int32_t *node,
+                                 const int32_t *code) {
+  if (!ctx) return FEMGPU_E_ARG;
+  BatchState *b = femgpu::ctx_batch(ctx);
+  if (!b) return femgpu::ctx_fail(ctx, FEMGPU_E_STATE, "no batch mesh");
+  if (!con_off) return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "null constraint offsets");
+  std::fill(b->fix.begin(), b->fix.end(), 0);
+  for (int i = 0; i < b->n_inst; ++i) {
+    const int64_t nn = b->node_off[i + 1] - b->node_off[i];
+    if (con_off[i + 1] < con_off[i]) return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "bad constraint offsets");
+    for (int64_t k = con_off[i]; k < con_off[i + 1]; ++k) {
+      if (node[k] < 0 || node[k] >= nn)
+        return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "constraint node out of range");
+      // bitwise decode of datakeeper.cpp:148-164
+      uint8_t m = 0;
+      if (code[k] & 1) m |= 1;
+      if (code[k] & 2) m |= 2;
+      if (b->dim == 3 && (code[k] & 3)) m |= 4;
+      b->fix[b->node_inv[b->node_off[i] + node[k]]] |= m;
+    }
+  }
+  uint8_t *df;
+  BCHK(femgpu::bup(b, femgpu::ctx_stream(ctx), "fix", b->fix, &df));
+  return FEMGPU_OK;
+}
+
+int femgpu_batch_solve(femgpu_ctx *ctx, int n_rhs, const double *f, const femgpu_solve_opts *opts,
+                       double *u_out, femgpu_batch_stat *stats, double *solve_ms) {
+  if (!ctx) return FEMGPU_E_ARG;
+  BatchState *b = femgpu::ctx_batch(ctx);
+  if (!b) return femgpu::ctx_fail(ctx, FEMGPU_E_STATE, "no batch mesh");
+  if (n_rhs < 1 || n_rhs > 4 || !f)
+    return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "n_rhs must be 1..4 and f non-null");
+  femgpu_solve_opts o;
+  femgpu_default_solve_opts(&o);
+  if (opts) o = *opts;
+  if (o.precision != FEMGPU_FP64 && o.precision != FEMGPU_MIXED)
+    return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "batch precision must be FP64 or MIXED");
+  if (o.tol_kind != FEMGPU_TOL_REL_RES || !(o.tol >= 0))
+    return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "batch solve needs FEMGPU_TOL_REL_RES and tol >= 0");
+  const int nr = n_rhs == 3 ? 4 : n_rhs;  // kernels exist for 1, 2, 4 vectors
+  const int dim = b->dim;
+  hipStream_t s = femgpu::ctx_stream(ctx);
+  BCHK(hipSetDevice(femgpu::ctx_device(ctx)));
+  const size_t nv = size_t(b->total_nodes) * nr * dim;
+  const size_t ns = size_t(b->n_slots) * nr * dim;
+  std::vector<double> fi(nv, 0.0);
+  for (int64_t g = 0; g < b->total_nodes; ++g) {
+    const int64_t u = b->node_perm[g];
+    for (int r = 0; r < n_rhs; ++r)
+      for (int c = 0; c < dim; ++c)
+        fi[(size_t(g) * nr + r) * dim + c] = f[(size_t(u) * n_rhs + r) * dim + c];
+  }
+  femgpu::BatchArgs<float> af{};
+  femgpu::BatchArgs<double> ad{};
+  double *d_f, *d_x, *d_y, *d_s64;
+  BStat *d_stat;
+  BCHK(femgpu::bup(b, s, "f", fi, &d_f));
+  BCHK(femgpu::bbuf(b, "x", nv, &d_x));
+  BCHK(femgpu::bbuf(b, "y64", nv, &d_y));
+  BCHK(femgpu::bbuf(b, "s64", ns, &d_s64));
+  BCHK(femgpu::bbuf(b, "stat", size_t(b->n_inst) * nr, &d_stat));
+  auto fill_common = [&](auto &a) {
+    a.inst = static_cast<const BInst *>(b->bufs["inst"].first);
+    a.order = static_cast<const int32_t *>(b->bufs["order"].first);
+    a.patches = static_cast<const femgpu::PatchInfo *>(b->bufs["patches"].first);
+    a.elem_lid = static_cast<const uint32_t *>(b->bufs["lid"].first);
+    a.epos = static_cast<const uint16_t *>(b->bufs["epos"].first);
+    a.node_end = static_cast<const uint16_t *>(b->bufs["nend"].first);
+    a.sh_gid = static_cast<const int32_t *>(b->bufs["shg"].first);
+    a.sh_slot = static_cast<const int32_t *>(b->bufs["shs"].first);
+    a.slot_start = static_cast<const int32_t *>(b->bufs["ss"].first);
+    a.fixmask = static_cast<const uint8_t *>(b->bufs["fix"].first);
+    a.coords64 = static_cast<const double *>(b->bufs["c64"].first);
+    a.f = d_f;
+    a.x = d_x;
+    a.y64 = d_y;
+    a.slots64 = d_s64;
+    a.stat = d_stat;
+    a.tol = o.tol;
+    a.drop = o.inner_tol > 0 && o.inner_tol < 1 ? o.inner_tol : 0.1;
+    a.max_iter = o.max_iter > 0 ? o.max_iter : 100000;
+    a.pn = b->pn;
+    a.emax = b->emax;
+  };
+  const bool mixed = o.precision == FEMGPU_MIXED;
+  if (mixed) {
+    fill_common(af);
+    af.coordsT = static_cast<const float *>(b->bufs["c32"].first);
+    BCHK(femgpu::bbuf(b, "r32", nv, &af.r));
+    BCHK(femgpu::bbuf(b, "p32", nv, &af.p));
+    BCHK(femgpu::bbuf(b, "q32", nv, &af.q));
+    BCHK(femgpu::bbuf(b, "s32", ns, &af.slotsT));
+    BCHK(femgpu::bbuf(b, "iD32", size_t(b->total_nodes) * dim, &af.invD));
+  } else {
+    fill_common(ad);
+    ad.coordsT = static_cast<const double *>(b->bufs["c64"].first);
+    BCHK(femgpu::bbuf(b, "r64", nv, &ad.r));
+    BCHK(femgpu::bbuf(b, "p64", nv, &ad.p));
+    BCHK(femgpu::bbuf(b, "q64", nv, &ad.q));
+    BCHK(femgpu::bbuf(b, "sT64", ns, &ad.slotsT));
+    BCHK(femgpu::bbuf(b, "iD64", size_t(b->total_nodes) * dim, &ad.invD));
+  }
+  const size_t lds = femgpu::batch_lds_bytes(dim, nr, b->pn, b->emax);
+  if (lds > 160 * 1024) return femgpu::ctx_fail(ctx, FEMGPU_E_UNSUPPORTED, "batch patches exceed LDS");
+  BCHK(hipEventRecord(b->ev0, s));
+  if (femgpu::launch_batch_pcg(dim, nr, mixed, af, ad, b->n_inst, lds, s) != 0)
+    return femgpu::ctx_fail(ctx, FEMGPU_E_UNSUPPORTED, "no batch kernel for this shape");
+  BCHK(hipGetLastError());
+  BCHK(hipEventRecord(b->ev1, s));
+  BCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (solve_ms) *solve_ms = ms;
+  std::vector<double> xi(nv);
+  std::vector<BStat> st(size_t(b->n_inst) * nr);
+  BCHK(hipMemcpyAsync(xi.data(), d_x, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  BCHK(hipMemcpyAsync(st.data(), d_stat, st.size() * sizeof(BStat), hipMemcpyDeviceToHost, s));
+  BCHK(hipStreamSynchronize(s));
+  if (u_out) {
+    for (int64_t g = 0; g < b->total_nodes; ++g) {
+      const int64_t u = b->node_perm[g];
+      for (int r = 0; r < n_rhs; ++r)
+        for (int c = 0; c < dim; ++c)
+          u_out[(size_t(u) * n_rhs + r) * dim + c] = xi[(size_t(g) * nr + r) * dim + c];
+    }
+  }
+  int worst = FEMGPU_OK;
+  for (int i = 0; i < b->n_inst; ++i)
+    for (int r = 0; r < n_rhs; ++r) {
+      const BStat &x = st[size_t(i) * nr + r];
+      if (stats) {
+        femgpu_batch_stat &o2 = stats[size_t(i) * n_rhs + r];
+        o2.status = x.status;
+        o2.iterations = x.iterations;
+        o2.replacements = x.replacements;
+        o2.rel_res = x.rel_res;
+      }
+      if (x.status != FEMGPU_OK && worst == FEMGPU_OK) worst = x.status;
+    }
+  if (worst != FEMGPU_OK)
+    return femgpu::ctx_fail(ctx, worst, "some batch systems did not converge (see stats)");
+  return FEMGPU_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "femgpu.h"
+#include "batch.h"
 #include "femgpu_internal.h"
 #include "kernels.h"
 
@@ -96,6 +97,7 @@ struct femgpu_ctx {
   int emax = 0;  // elements per patch (0: default)
 
   bool have_mesh = false;
+  femgpu::BatchState *batch = nullptr;  // batched solve (batch.cpp)
   MeshPlan plan;
   DevMesh dm;
   std::map<std::string, DevBuf> bufs;
@@ -115,6 +117,16 @@ struct femgpu_ctx {
   uint64_t buf_gen = 0;
   uint64_t graph_gen = 0;
 };
+
+namespace femgpu {
+hipStream_t ctx_stream(femgpu_ctx *c) { return c->stream; }
+int ctx_device(femgpu_ctx *c) { return c->device; }
+int ctx_fail(femgpu_ctx *c, int code, const char *msg) {
+  if (c) c->err = msg;
+  return code;
+}
+BatchState *&ctx_batch(femgpu_ctx *c) { return c->batch; }
+}  // namespace femgpu
 
 namespace {
 
@@ -537,6 +549,7 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
   HIPCHK(femgpu::prepare_kernels());
+  femgpu::batch_prepare();
   *out = c.release();
   return FEMGPU_OK;
 }
@@ -546,6 +559,7 @@ void femgpu_destroy(femgpu_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graphs(ctx);
+  femgpu::batch_free(ctx->batch);
   for (auto &kv : ctx->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
   if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);

@@ -26,6 +26,7 @@
 // run to run for a given mesh plan.
 #include <hip/hip_runtime.h>
 
+#include "element.h"
 #include "kernels.h"
 
 namespace femgpu {
@@ -54,201 +55,7 @@ __host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ema
   return L;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Block-wide sum, same value returned in every thread; fixed order.
-template <int BT = NT>
-__device__ __forceinline__ double block_sum(double v, double *sh) {
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
-  if (l == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0;
-#pragma unroll
-  for (int i = 0; i < BT / 64; ++i) r += sh[i];
-  return r;
-}
-
-// ---------------------------------------------------------------------------
-// Element kernels (registers only).  X, U: node-major (npe x dim).
-// fix: bit (dim*a + c) set when that local DOF is constrained.
-// Returns u'K u of the (constraint-modified) element matrix.
-
-template <typename T>
-__device__ __forceinline__ T tet_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
-                                       T *F) {
-  const T e1x = X[3] - X[0], e1y = X[4] - X[1], e1z = X[5] - X[2];
-  const T e2x = X[6] - X[0], e2y = X[7] - X[1], e2z = X[8] - X[2];
-  const T e3x = X[9] - X[0], e3y = X[10] - X[1], e3z = X[11] - X[2];
-  T c[4][3];
-  c[1][0] = e2y * e3z - e2z * e3y;
-  c[1][1] = e2z * e3x - e2x * e3z;
-  c[1][2] = e2x * e3y - e2y * e3x;
-  c[2][0] = e3y * e1z - e3z * e1y;
-  c[2][1] = e3z * e1x - e3x * e1z;
-  c[2][2] = e3x * e1y - e3y * e1x;
-  c[3][0] = e1y * e2z - e1z * e2y;
-  c[3][1] = e1z * e2x - e1x * e2z;
-  c[3][2] = e1x * e2y - e1y * e2x;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) c[0][j] = -(c[1][j] + c[2][j] + c[3][j]);
-  const T det = e1x * c[1][0] + e1y * c[1][1] + e1z * c[1][2];
-  const T s = es / (T(6) * fabs(det));
-  T um[12];
-#pragma unroll
-  for (int i = 0; i < 12; ++i) um[i] = ((fix >> i) & 1u) ? T(0) : U[i];
-  T H[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) H[i][j] = T(0);
-#pragma unroll
-  for (int a = 1; a < 4; ++a)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const T du = um[3 * a + i] - um[i];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) H[i][j] += du * c[a][j];
-    }
-  const T tr = H[0][0] + H[1][1] + H[2][2];
-  const T S00 = lam * tr + T(2) * mu * H[0][0];
-  const T S11 = lam * tr + T(2) * mu * H[1][1];
-  const T S22 = lam * tr + T(2) * mu * H[2][2];
-  const T S01 = mu * (H[0][1] + H[1][0]);
-  const T S02 = mu * (H[0][2] + H[2][0]);
-  const T S12 = mu * (H[1][2] + H[2][1]);
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    F[3 * a + 0] = s * (S00 * c[a][0] + S01 * c[a][1] + S02 * c[a][2]);
-    F[3 * a + 1] = s * (S01 * c[a][0] + S11 * c[a][1] + S12 * c[a][2]);
-    F[3 * a + 2] = s * (S02 * c[a][0] + S12 * c[a][1] + S22 * c[a][2]);
-  }
-  if (fix) {
-#pragma unroll
-    for (int i = 0; i < 12; ++i)
-      if ((fix >> i) & 1u) {
-        const int a = i / 3, cc = i % 3;
-        const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1] + c[a][2] * c[a][2];
-        F[i] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2) * U[i];
-      }
-  }
-  T en = T(0);
-#pragma unroll
-  for (int i = 0; i < 12; ++i) en += U[i] * F[i];
-  return en;
-}
-
-template <typename T>
-__device__ __forceinline__ T tri_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
-                                       T *F) {
-  const T e1x = X[2] - X[0], e1y = X[3] - X[1];
-  const T e2x = X[4] - X[0], e2y = X[5] - X[1];
-  T c[3][2];
-  c[1][0] = e2y;
-  c[1][1] = -e2x;
-  c[2][0] = -e1y;
-  c[2][1] = e1x;
-  c[0][0] = -(c[1][0] + c[2][0]);
-  c[0][1] = -(c[1][1] + c[2][1]);
-  const T det = e1x * e2y - e1y * e2x;
-  const T s = es / (T(2) * fabs(det));
-  T um[6];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) um[i] = ((fix >> i) & 1u) ? T(0) : U[i];
-  T H00 = T(0), H01 = T(0), H10 = T(0), H11 = T(0);
-#pragma unroll
-  for (int a = 1; a < 3; ++a) {
-    const T dx = um[2 * a] - um[0], dy = um[2 * a + 1] - um[1];
-    H00 += dx * c[a][0];
-    H01 += dx * c[a][1];
-    H10 += dy * c[a][0];
-    H11 += dy * c[a][1];
-  }
-  const T tr = H00 + H11;
-  const T S00 = lam * tr + T(2) * mu * H00;
-  const T S11 = lam * tr + T(2) * mu * H11;
-  const T S01 = mu * (H01 + H10);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    F[2 * a + 0] = s * (S00 * c[a][0] + S01 * c[a][1]);
-    F[2 * a + 1] = s * (S01 * c[a][0] + S11 * c[a][1]);
-  }
-  if (fix) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-      if ((fix >> i) & 1u) {
-        const int a = i / 2, cc = i % 2;
-        const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1];
-        F[i] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2) * U[i];
-      }
-  }
-  T en = T(0);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) en += U[i] * F[i];
-  return en;
-}
-
-template <typename T, int DIM>
-__device__ __forceinline__ T elem_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
-                                        T *F) {
-  if constexpr (DIM == 3)
-    return tet_force<T>(X, U, lam, mu, es, fix, F);
-  else
-    return tri_force<T>(X, U, lam, mu, es, fix, F);
-}
-
-// Diagonal entries K_ii = s((lam+mu) c_ai^2 + mu |c_a|^2).
-template <typename T, int DIM>
-__device__ __forceinline__ void elem_diag(const T *X, T lam, T mu, T es, T *F) {
-  if constexpr (DIM == 3) {
-    const T e1x = X[3] - X[0], e1y = X[4] - X[1], e1z = X[5] - X[2];
-    const T e2x = X[6] - X[0], e2y = X[7] - X[1], e2z = X[8] - X[2];
-    const T e3x = X[9] - X[0], e3y = X[10] - X[1], e3z = X[11] - X[2];
-    T c[4][3];
-    c[1][0] = e2y * e3z - e2z * e3y;
-    c[1][1] = e2z * e3x - e2x * e3z;
-    c[1][2] = e2x * e3y - e2y * e3x;
-    c[2][0] = e3y * e1z - e3z * e1y;
-    c[2][1] = e3z * e1x - e3x * e1z;
-    c[2][2] = e3x * e1y - e3y * e1x;
-    c[3][0] = e1y * e2z - e1z * e2y;
-    c[3][1] = e1z * e2x - e1x * e2z;
-    c[3][2] = e1x * e2y - e1y * e2x;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) c[0][j] = -(c[1][j] + c[2][j] + c[3][j]);
-    const T det = e1x * c[1][0] + e1y * c[1][1] + e1z * c[1][2];
-    const T s = es / (T(6) * fabs(det));
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1] + c[a][2] * c[a][2];
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) F[3 * a + cc] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2);
-    }
-  } else {
-    const T e1x = X[2] - X[0], e1y = X[3] - X[1];
-    const T e2x = X[4] - X[0], e2y = X[5] - X[1];
-    T c[3][2];
-    c[1][0] = e2y;
-    c[1][1] = -e2x;
-    c[2][0] = -e1y;
-    c[2][1] = e1x;
-    c[0][0] = -(c[1][0] + c[2][0]);
-    c[0][1] = -(c[1][1] + c[2][1]);
-    const T det = e1x * e2y - e1y * e2x;
-    const T s = es / (T(2) * fabs(det));
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const T n2 = c[a][0] * c[a][0] + c[a][1] * c[a][1];
-#pragma unroll
-      for (int cc = 0; cc < 2; ++cc) F[2 * a + cc] = s * ((lam + mu) * c[a][cc] * c[a][cc] + mu * n2);
-    }
-  }
-}
+using namespace elem;
 
 // ---------------------------------------------------------------------------
 // Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step (stop test and beta

@@ -252,6 +252,77 @@ def cantilever_problem(nx=96, ny=32, nz=32, lx=3.0, ly=1.0, lz=1.0, E0=2e7, nu=0
     return pb
 
 
+@dataclasses.dataclass
+class BatchInstance:
+    """One plate of a batch: mesh, material, constraints and load cases
+    f (n_nodes, n_rhs, dim), assembled consistent nodal loads."""
+    nodes: np.ndarray
+    elems: np.ndarray
+    E: float
+    nu: float
+    constr_node: np.ndarray
+    constr_code: np.ndarray
+    f: np.ndarray
+
+    def problem(self) -> Problem:
+        """The instance as a single Problem (for per-instance reference solves)."""
+        return Problem(nodes=self.nodes, elems=self.elems, constr_node=self.constr_node,
+                       constr_code=self.constr_code, E=self.E, nu=self.nu)
+
+
+def _edge_load(nodes, on_edge, axis_along, comp, t, f):
+    """Consistent nodal loads of a uniform traction t (component `comp`) on the
+    straight edge made of the nodes `on_edge`: t * h / 2 to both ends of every
+    segment."""
+    idx = on_edge[np.argsort(nodes[on_edge, axis_along])]
+    seg = np.diff(nodes[idx, axis_along])
+    f[idx[:-1], comp] += t * seg / 2.0
+    f[idx[1:], comp] += t * seg / 2.0
+
+
+def plate_batch(n=512, seed=2024, nx_range=(16, 160), ny_range=(16, 80), lx=1.0, ly=0.5,
+                traction=1e3, body=1e3):
+    """Batch of 2D plates with seeded uneven grids (nx ~ U{nx_range},
+    ny ~ U{ny_range}), E ~ LogNormal(ln 2e7, 0.5), nu ~ U[0.2, 0.35]; left edge
+    fixed (code 3); four load cases: x-traction and y-traction on the right
+    edge, shear (x-traction) on the top edge, body force -body per unit area
+    in y (rho g A / 3 to each node).  Loads on fixed DOFs are zero."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        E = float(rng.lognormal(np.log(2e7), 0.5))
+        nu = float(rng.uniform(0.2, 0.35))
+        nodes, elems = plate_tri(nx, ny, lx, ly)
+        nn = len(nodes)
+        left = nodes_on_plane(nodes, 0, 0.0)
+        right = nodes_on_plane(nodes, 0, lx)
+        top = nodes_on_plane(nodes, 1, ly)
+        f = np.zeros((nn, 4, 2))
+        fx, fy, fs, fb = (np.zeros((nn, 2)) for _ in range(4))
+        _edge_load(nodes, right, 1, 0, traction, fx)
+        _edge_load(nodes, right, 1, 1, traction, fy)
+        _edge_load(nodes, top, 0, 0, traction, fs)
+        X = nodes[elems]
+        A = 0.5 * np.abs((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1])
+                         - (X[:, 1, 1] - X[:, 0, 1]) * (X[:, 2, 0] - X[:, 0, 0]))
+        np.add.at(fb[:, 1], elems.ravel(), np.repeat(-body * A / 3.0, 3))
+        for k, fk in enumerate((fx, fy, fs, fb)):
+            f[:, k] = fk
+        f[left] = 0.0
+        out.append(BatchInstance(nodes=nodes, elems=elems, E=E, nu=nu, constr_node=left,
+                                 constr_code=np.full(len(left), 3, np.int32), f=f))
+    return out
+
+
+def load_batch(ctx, insts):
+    """Load a list of BatchInstance into a Context's batch."""
+    ctx.batch_set_mesh([(b.nodes, b.elems) for b in insts])
+    ctx.batch_set_material([b.E for b in insts], [b.nu for b in insts])
+    ctx.batch_set_constraints([(b.constr_node, b.constr_code) for b in insts])
+
+
 # --------------------------------------------------------------------------
 # prepared-mesh files
 

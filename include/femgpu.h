@@ -196,6 +196,43 @@ int femgpu_material_grad(femgpu_ctx *ctx, const double *u, const double *w, doub
 int femgpu_compliance_grad(femgpu_ctx *ctx, const femgpu_solve_opts *opts, double *u_out,
                            double *c_out, double *g_out, femgpu_stats *stats);
 
+/* --- batched independent static solves ----------------------------------------
+ * No counterpart in the reference (it solves one problem per process run).
+ * Many small meshes ("instances"), each with its own material, constraints
+ * and 1..4 right-hand sides, solved in ONE persistent launch: one workgroup
+ * runs an instance's whole Jacobi-PCG (the algorithm of gpuPCG_EbE_vec,
+ * femfunc.cpp:25-128, with the relative-residual stopping rule), largest
+ * instances first.  Meshes are concatenated: instance i owns user nodes
+ * [node_off[i], node_off[i+1]) and elements [elem_off[i], elem_off[i+1]);
+ * element node ids and constraint node ids are instance-local. */
+typedef struct {
+  int status;        /* FEMGPU_OK or the reason the system stopped          */
+  int iterations;    /* PCG iterations                                       */
+  int replacements;  /* FP64 residual replacements (MIXED)                   */
+  int pad;
+  double rel_res;    /* ||f - K u||_2 / ||f||_2 in FP64 at the end            */
+} femgpu_batch_stat;
+
+/* nodes: node_off[n_inst] x dim; elems: elem_off[n_inst] x (dim+1).
+ * Replaces the instance's plan; material defaults to E = 2e7, nu = 0.3. */
+int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *node_off,
+                          const double *nodes, const int64_t *elem_off, const int32_t *elems);
+/* Per-instance Young's modulus and Poisson ratio (n_inst each). */
+int femgpu_batch_set_material(femgpu_ctx *ctx, const double *E, const double *nu);
+/* Constraint lines "node code" per instance (datakeeper.cpp:139-171 bitwise
+ * codes): instance i holds entries [con_off[i], con_off[i+1]). */
+int femgpu_batch_set_constraints(femgpu_ctx *ctx, const int64_t *con_off, const int32_t *node,
+                                 const int32_t *code);
+/* Solve every instance for n_rhs (1..4) right-hand sides.
+ * f, u_out: node_off[n_inst] x n_rhs x dim (node-major, then rhs, then
+ * component).  opts: precision FP64 or MIXED, tol_kind FEMGPU_TOL_REL_RES,
+ * tol, max_iter (0: 100000), inner_tol (MIXED residual drop, default 0.1).
+ * stats: n_inst x n_rhs (may be NULL); solve_ms: device time (may be NULL).
+ * Returns FEMGPU_OK when every system converged, else the first failing
+ * system's status (u_out and stats are filled either way). */
+int femgpu_batch_solve(femgpu_ctx *ctx, int n_rhs, const double *f, const femgpu_solve_opts *opts,
+                       double *u_out, femgpu_batch_stat *stats, double *solve_ms);
+
 /* --- per-element results ----------------------------------------------------
  * No counterpart in the reference's live code; the golden fixtures hold
  * sigma = D B u_e per element (testing/<task>_test/results.vtk, TENSORS stress).

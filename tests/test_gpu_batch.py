@@ -1,0 +1,114 @@
+"""Batched independent solves (femgpu_batch_*): every (instance, right-hand
+side) against a dense FP64 solve of the oracle's assembled operator for that
+instance, plus padding, determinism, zero right-hand sides, 3D instances and
+the max-iteration status.  Tolerances are stated per test."""
+import numpy as np
+import pytest
+
+import cudafemproject_amd as F
+from cudafemproject_amd._lib import E_MAXIT
+from cudafemproject_amd.problem import (BatchInstance, Problem, box_tet, load_batch,
+                                        nodes_on_plane, plate_batch)
+from helpers import dense_K
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def small_plates(n, seed, nr=4):
+    insts = plate_batch(n, seed=seed, nx_range=(2, 9), ny_range=(2, 6))
+    for b in insts:
+        b.f = b.f[:, :nr]
+    return insts
+
+
+def dense_solution(b):
+    pb = b.problem()
+    K = dense_K(pb)
+    nr = b.f.shape[1]
+    return np.stack([np.linalg.solve(K, b.f[:, r].ravel()).reshape(-1, pb.dim) for r in range(nr)], 1)
+
+
+@pytest.mark.parametrize("precision,tol,bound", [(F.FP64, 1e-12, 1e-9), (F.MIXED, 1e-10, 1e-7)])
+def test_batch_vs_dense(ctx, precision, tol, bound):
+    insts = small_plates(9, seed=3)
+    print(f"[batch vs dense] prec={precision} n_inst={len(insts)}", flush=True)
+    load_batch(ctx, insts)
+    us, st, ms = ctx.batch_solve([b.f for b in insts], precision=precision, tol=tol)
+    assert np.all(st["status"] == 0)
+    assert np.all(st["rel_res"] <= tol)
+    assert np.all(st["iterations"] > 0)
+    for b, u in zip(insts, us):
+        assert rel(u, dense_solution(b)) < bound
+
+
+@pytest.mark.parametrize("nr", [1, 2, 3])
+def test_batch_rhs_counts(ctx, nr):
+    insts = small_plates(5, seed=7, nr=nr)
+    load_batch(ctx, insts)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-12)
+    assert st.shape == (5, nr) and np.all(st["status"] == 0)
+    for b, u in zip(insts, us):
+        assert u.shape == b.f.shape
+        assert rel(u, dense_solution(b)) < 1e-9
+
+
+def test_batch_deterministic_and_zero_rhs(ctx):
+    insts = small_plates(6, seed=11)
+    insts[2].f[:] = 0.0                     # an instance with no load at all
+    insts[4].f[:, 1] = 0.0                  # one load case of another
+    load_batch(ctx, insts)
+    u1, s1, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-9)
+    u2, s2, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-9)
+    for a, b in zip(u1, u2):
+        assert np.array_equal(a, b)
+    assert np.array_equal(s1, s2)
+    assert np.all(u1[2] == 0.0) and np.all(s1["iterations"][2] == 0)
+    assert np.all(u1[4][:, 1] == 0.0) and s1["iterations"][4, 1] == 0
+    assert np.all(s1["status"] == 0)
+
+
+def test_batch_3d(ctx):
+    insts = []
+    rng = np.random.default_rng(5)
+    for i in range(4):
+        n = (2 + i, 2, 2)
+        nodes, elems = box_tet(*n, 1.0 + 0.5 * i, 1.0, 1.0)
+        fixed = nodes_on_plane(nodes, 0, 0.0)
+        f = rng.standard_normal((len(nodes), 2, 3))
+        f[fixed] = 0.0
+        insts.append(BatchInstance(nodes=nodes, elems=elems, E=float(rng.uniform(1e7, 3e7)),
+                                   nu=float(rng.uniform(0.2, 0.35)), constr_node=fixed,
+                                   constr_code=np.full(len(fixed), 7, np.int32), f=f))
+    load_batch(ctx, insts)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-12)
+    assert np.all(st["status"] == 0)
+    for b, u in zip(insts, us):
+        assert rel(u, dense_solution(b)) < 1e-9
+
+
+def test_batch_max_iter_status(ctx):
+    insts = small_plates(3, seed=13)
+    load_batch(ctx, insts)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-14, max_iter=3,
+                                raise_on_error=False)
+    assert np.all(st["status"] == E_MAXIT)
+    assert np.all(st["iterations"] == 3)
+
+
+def test_batch_matches_single_instance_path(ctx):
+    """Plates of the benchmark's size range against the single-problem solver
+    (femgpu_solve_rhs) on the same instance and load case."""
+    insts = plate_batch(6, seed=21, nx_range=(16, 60), ny_range=(16, 40))
+    load_batch(ctx, insts)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-10)
+    assert np.all(st["status"] == 0)
+    for b, u in zip(insts, us):
+        ctx.set_problem(b.problem())
+        for r in range(4):
+            us1, s1 = ctx.solve_rhs(b.f[:, r], F.FP64, tol=1e-12)
+            assert rel(u[:, r], us1) < 1e-8
