@@ -268,7 +268,7 @@ def bench_batch(args, F):
     for b, it in zip(insts, smx["iterations"].max(1)):
         ne, nn = len(b.elems), len(b.nodes)
         N = 2 * nn
-        tot += it * (4 * 3 * ne + 4 * 2 * nn + 2 * 4 * N * 4 + 11 * 4 * N * 4)
+        tot += int(it) * (4 * 3 * ne + 4 * 2 * nn + 2 * 4 * N * 4 + 11 * 4 * N * 4)
     ach = tot / (tmx * 1e-3) / 1e9
     line = {
         "metric": "batched static solves: time for 512 plates x 4 load cases to rel-res 1e-8",

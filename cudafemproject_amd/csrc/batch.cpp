@@ -1,6 +1,7 @@
 // batch.cpp -- host side of the batched solve (femgpu_batch_* in femgpu.h):
-// plans every instance, concatenates the plans into one device layout
-// (batch.h) and launches the persistent PCG kernel (batch.hip).
+// plans every instance, turns the plans into the flat per-element and
+// per-patch-node tables of batch.h and launches the persistent PCG kernel
+// (batch.hip).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,7 +25,6 @@ struct BatchState {
   std::vector<int32_t> node_inv;   // user global node -> internal global node
   std::vector<int32_t> order;
   std::vector<uint8_t> fix;        // per internal global node
-  int pn = 0, emax = 0;
   int64_t n_slots = 0;
   std::map<std::string, std::pair<void *, size_t>> bufs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -117,7 +117,7 @@ int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *n
   bp = nullptr;
   auto *b = new BatchState();
   const int npe = dim + 1;
-  const int emax_b = dim == 2 ? 256 : 128;
+  const int emax_b = 512;
   if (node_off[0] != 0 || elem_off[0] != 0) {
     delete b;
     return femgpu::ctx_fail(ctx, FEMGPU_E_ARG, "batch offsets must start at 0");
@@ -141,10 +141,7 @@ int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *n
   b->node_perm.assign(b->total_nodes, 0);
   b->node_inv.assign(b->total_nodes, 0);
   b->fix.assign(b->total_nodes, 0);
-  std::vector<femgpu::PatchInfo> patches;
-  std::vector<uint32_t> lid;
-  std::vector<uint16_t> epos, nend;
-  std::vector<int32_t> shg, shs, ss;
+  std::vector<int32_t> bel, bpos, pbeg, pend, ptgt, ss;
   std::vector<double> c64(size_t(b->total_nodes) * dim);
   int64_t e_int = 0;
   for (int i = 0; i < n_inst; ++i) {
@@ -173,21 +170,23 @@ int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *n
       return femgpu::ctx_fail(ctx, FEMGPU_E_BAD_MESH, ("instance " + std::to_string(i) + ": " + msg).c_str());
     }
     const int32_t N0 = int32_t(node_off[i]);
-    const int32_t P0 = int32_t(patches.size());
-    const int32_t SH0 = int32_t(shg.size()), PN0 = int32_t(nend.size());
+    const int32_t PN0 = int32_t(pbeg.size());
     const int32_t S0 = int32_t(b->n_slots), IF0 = int32_t(ss.size());
-    for (femgpu::PatchInfo pi : P.patches) {
-      pi.elem_start += int32_t(e_int);
-      pi.own_start += N0;
-      pi.sh_start += SH0;
-      pi.pnode_start += PN0;
-      patches.push_back(pi);
+    bel.insert(bel.end(), P.elems_int.begin(), P.elems_int.end());
+    const size_t pos0 = bpos.size();
+    bpos.resize(pos0 + size_t(ne) * npe);
+    for (const femgpu::PatchInfo &pi : P.patches) {
+      const int32_t base = pi.elem_start * npe;  // the patch's entries follow its elements
+      for (int32_t e = pi.elem_start; e < pi.elem_start + pi.n_elem; ++e)
+        for (int a = 0; a < npe; ++a) bpos[pos0 + size_t(e) * npe + a] = base + P.epos[size_t(e) * 4 + a];
+      for (int32_t k = 0; k < pi.n_own + pi.n_sh; ++k) {
+        const int32_t e1 = P.node_end[pi.pnode_start + k];
+        const int32_t e0 = k ? int32_t(P.node_end[pi.pnode_start + k - 1]) : 0;
+        pbeg.push_back(base + e0);
+        pend.push_back(base + e1);
+        ptgt.push_back(k < pi.n_own ? pi.own_start + k : -1 - P.sh_slot[pi.sh_start + k - pi.n_own]);
+      }
     }
-    lid.insert(lid.end(), P.elem_lid.begin(), P.elem_lid.end());
-    epos.insert(epos.end(), P.epos.begin(), P.epos.end());
-    nend.insert(nend.end(), P.node_end.begin(), P.node_end.end());
-    for (int32_t g : P.sh_gid) shg.push_back(g + N0);
-    for (int32_t sl : P.sh_slot) shs.push_back(sl + S0);
     for (int32_t st : P.slot_start) ss.push_back(st + S0);
     for (int64_t j = 0; j < nn; ++j) {
       b->node_perm[N0 + j] = N0 + P.node_perm[j];
@@ -196,20 +195,20 @@ int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *n
         c64[size_t(N0 + j) * dim + c] = nd[size_t(P.node_perm[j]) * dim + c];
     }
     BInst I{};
-    I.patch_begin = P0;
-    I.patch_end = int32_t(patches.size());
     I.node_off = N0;
     I.n_nodes = int32_t(nn);
     I.n_owned = int32_t(P.n_owned);
     I.n_active = int32_t(P.n_owned + P.n_iface);
+    I.elem_off = int32_t(e_int);
+    I.n_elems = int32_t(ne);
+    I.pn_off = PN0;
+    I.n_pn = int32_t(pbeg.size()) - PN0;
     I.iface_off = IF0;
     I.slot_off = S0;
     const double E = 2e7, nu = 0.3;
     I.mu = E / (2.0 * (1.0 + nu));
     I.lam = dim == 3 ? E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)) : E * nu / (1.0 - nu * nu);
     b->inst.push_back(I);
-    b->pn = std::max(b->pn, P.pn_used);
-    b->emax = std::max(b->emax, P.emax_used);
     b->n_slots += P.n_slots;
     e_int += ne;
   }
@@ -220,22 +219,18 @@ int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *n
     return b->elem_off[a + 1] - b->elem_off[a] > b->elem_off[c + 1] - b->elem_off[c];
   });
   std::vector<float> c32(c64.begin(), c64.end());
-  femgpu::PatchInfo *dp;
-  uint32_t *dl;
-  uint16_t *de, *dn;
-  int32_t *dg, *dsl, *dss, *dor;
+  int32_t *p0, *p1, *p2, *p3, *p4, *p5, *p6;
   double *dc64;
   float *dc32;
   BInst *di;
   uint8_t *df;
-  BCHK(femgpu::bup(b, s, "patches", patches, &dp));
-  BCHK(femgpu::bup(b, s, "lid", lid, &dl));
-  BCHK(femgpu::bup(b, s, "epos", epos, &de));
-  BCHK(femgpu::bup(b, s, "nend", nend, &dn));
-  BCHK(femgpu::bup(b, s, "shg", shg, &dg));
-  BCHK(femgpu::bup(b, s, "shs", shs, &dsl));
-  BCHK(femgpu::bup(b, s, "ss", ss, &dss));
-  BCHK(femgpu::bup(b, s, "order", b->order, &dor));
+  BCHK(femgpu::bup(b, s, "elems", bel, &p0));
+  BCHK(femgpu::bup(b, s, "gpos", bpos, &p1));
+  BCHK(femgpu::bup(b, s, "pbeg", pbeg, &p2));
+  BCHK(femgpu::bup(b, s, "pend", pend, &p3));
+  BCHK(femgpu::bup(b, s, "ptgt", ptgt, &p4));
+  BCHK(femgpu::bup(b, s, "ss", ss, &p5));
+  BCHK(femgpu::bup(b, s, "order", b->order, &p6));
   BCHK(femgpu::bup(b, s, "c64", c64, &dc64));
   BCHK(femgpu::bup(b, s, "c32", c32, &dc32));
   BCHK(femgpu::bup(b, s, "inst", b->inst, &di));
@@ -326,17 +321,19 @@ int femgpu_batch_solve(femgpu_ctx *ctx, int n_rhs, const double *f, const femgpu
   BCHK(femgpu::bbuf(b, "y64", nv, &d_y));
   BCHK(femgpu::bbuf(b, "s64", ns, &d_s64));
   BCHK(femgpu::bbuf(b, "stat", size_t(b->n_inst) * nr, &d_stat));
+  double *d_ent;
+  BCHK(femgpu::bbuf(b, "ent", size_t(b->total_elems) * (dim + 1) * nr * dim, &d_ent));
   auto fill_common = [&](auto &a) {
     a.inst = static_cast<const BInst *>(b->bufs["inst"].first);
     a.order = static_cast<const int32_t *>(b->bufs["order"].first);
-    a.patches = static_cast<const femgpu::PatchInfo *>(b->bufs["patches"].first);
-    a.elem_lid = static_cast<const uint32_t *>(b->bufs["lid"].first);
-    a.epos = static_cast<const uint16_t *>(b->bufs["epos"].first);
-    a.node_end = static_cast<const uint16_t *>(b->bufs["nend"].first);
-    a.sh_gid = static_cast<const int32_t *>(b->bufs["shg"].first);
-    a.sh_slot = static_cast<const int32_t *>(b->bufs["shs"].first);
+    a.elems = static_cast<const int32_t *>(b->bufs["elems"].first);
+    a.gpos = static_cast<const int32_t *>(b->bufs["gpos"].first);
+    a.pn_beg = static_cast<const int32_t *>(b->bufs["pbeg"].first);
+    a.pn_end = static_cast<const int32_t *>(b->bufs["pend"].first);
+    a.pn_tgt = static_cast<const int32_t *>(b->bufs["ptgt"].first);
     a.slot_start = static_cast<const int32_t *>(b->bufs["ss"].first);
     a.fixmask = static_cast<const uint8_t *>(b->bufs["fix"].first);
+    a.ent = d_ent;
     a.coords64 = static_cast<const double *>(b->bufs["c64"].first);
     a.f = d_f;
     a.x = d_x;
@@ -346,8 +343,6 @@ int femgpu_batch_solve(femgpu_ctx *ctx, int n_rhs, const double *f, const femgpu
     a.tol = o.tol;
     a.drop = o.inner_tol > 0 && o.inner_tol < 1 ? o.inner_tol : 0.1;
     a.max_iter = o.max_iter > 0 ? o.max_iter : 100000;
-    a.pn = b->pn;
-    a.emax = b->emax;
   };
   const bool mixed = o.precision == FEMGPU_MIXED;
   if (mixed) {
@@ -367,10 +362,8 @@ int femgpu_batch_solve(femgpu_ctx *ctx, int n_rhs, const double *f, const femgpu
     BCHK(femgpu::bbuf(b, "sT64", ns, &ad.slotsT));
     BCHK(femgpu::bbuf(b, "iD64", size_t(b->total_nodes) * dim, &ad.invD));
   }
-  const size_t lds = femgpu::batch_lds_bytes(dim, nr, b->pn, b->emax);
-  if (lds > 160 * 1024) return femgpu::ctx_fail(ctx, FEMGPU_E_UNSUPPORTED, "batch patches exceed LDS");
   BCHK(hipEventRecord(b->ev0, s));
-  if (femgpu::launch_batch_pcg(dim, nr, mixed, af, ad, b->n_inst, lds, s) != 0)
+  if (femgpu::launch_batch_pcg(dim, nr, mixed, af, ad, b->n_inst, s) != 0)
     return femgpu::ctx_fail(ctx, FEMGPU_E_UNSUPPORTED, "no batch kernel for this shape");
   BCHK(hipGetLastError());
   BCHK(hipEventRecord(b->ev1, s));

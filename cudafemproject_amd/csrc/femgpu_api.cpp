@@ -549,7 +549,6 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
   HIPCHK(femgpu::prepare_kernels());
-  femgpu::batch_prepare();
   *out = c.release();
   return FEMGPU_OK;
 }

@@ -80,8 +80,9 @@ def test_compliance_grad_central_fd(ctx):
     err = np.abs(fd - dC_drho[sample])
     print(f"  max |adj-fd| / max|fd| = {err.max() / np.abs(fd).max():.3e}, "
           f"max per-element {np.max(err / np.abs(fd)):.3e}", flush=True)
+    # the FD's own noise is ~1e-11 absolute (rounding of the individual terms)
     assert err.max() <= 1e-5 * np.abs(fd).max()
-    assert np.all(err <= 1e-5 * np.abs(fd))
+    assert np.all(err <= 1e-5 * np.abs(fd) + 1e-8 * np.abs(fd).max())
 
 
 def test_compliance_grad_full_cantilever(ctx):
