@@ -164,7 +164,7 @@ int femgpu_batch_set_mesh(femgpu_ctx *ctx, int dim, int n_inst, const int64_t *n
       }
     }
     femgpu::MeshPlan P;
-    const std::string msg = femgpu::build_plan(dim, nn, nd, ne, el, emax_b, P);
+    const std::string msg = femgpu::build_plan(dim, nn, nd, ne, el, emax_b, femgpu::PN_LIMIT, false, P);
     if (!msg.empty()) {
       delete b;
       return femgpu::ctx_fail(ctx, FEMGPU_E_BAD_MESH, ("instance " + std::to_string(i) + ": " + msg).c_str());

@@ -157,6 +157,102 @@ __device__ __forceinline__ T tri_force(const T *X, const T *U, T lam, T mu, T es
   return en;
 }
 
+// 1/x: the hardware reciprocal (1 ulp) in FP32, a correctly rounded
+// division in FP64.
+template <typename T>
+__device__ __forceinline__ T recip(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rcpf(x);
+  else
+    return T(1) / x;
+}
+
+// Element force of an unconstrained element block (the patch kernels zero
+// constrained components of u per node and set the constrained rows from
+// the assembled diagonal): f_a = S(H)' c_a with S(H) scaled by
+// es / (6|det J|) once, and f_0 = -(f_1 + f_2 + f_3) since c_0 = -sum c_a.
+// Returns u'K u of the element.
+template <typename T>
+__device__ __forceinline__ T tet_force_free(const T *X, const T *U, T lam, T mu, T es, T *F) {
+  const T e1x = X[3] - X[0], e1y = X[4] - X[1], e1z = X[5] - X[2];
+  const T e2x = X[6] - X[0], e2y = X[7] - X[1], e2z = X[8] - X[2];
+  const T e3x = X[9] - X[0], e3y = X[10] - X[1], e3z = X[11] - X[2];
+  T c[4][3];
+  c[1][0] = e2y * e3z - e2z * e3y;
+  c[1][1] = e2z * e3x - e2x * e3z;
+  c[1][2] = e2x * e3y - e2y * e3x;
+  c[2][0] = e3y * e1z - e3z * e1y;
+  c[2][1] = e3z * e1x - e3x * e1z;
+  c[2][2] = e3x * e1y - e3y * e1x;
+  c[3][0] = e1y * e2z - e1z * e2y;
+  c[3][1] = e1z * e2x - e1x * e2z;
+  c[3][2] = e1x * e2y - e1y * e2x;
+  const T det = e1x * c[1][0] + e1y * c[1][1] + e1z * c[1][2];
+  const T s = es * recip<T>(T(6) * fabs(det));
+  T H[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T d1 = U[3 + i] - U[i], d2 = U[6 + i] - U[i], d3 = U[9 + i] - U[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) H[i][j] = d1 * c[1][j] + d2 * c[2][j] + d3 * c[3][j];
+  }
+  const T ls = lam * s, ms = mu * s;
+  const T tr = ls * (H[0][0] + H[1][1] + H[2][2]);
+  const T S00 = tr + T(2) * ms * H[0][0];
+  const T S11 = tr + T(2) * ms * H[1][1];
+  const T S22 = tr + T(2) * ms * H[2][2];
+  const T S01 = ms * (H[0][1] + H[1][0]);
+  const T S02 = ms * (H[0][2] + H[2][0]);
+  const T S12 = ms * (H[1][2] + H[2][1]);
+#pragma unroll
+  for (int a = 1; a < 4; ++a) {
+    F[3 * a + 0] = S00 * c[a][0] + S01 * c[a][1] + S02 * c[a][2];
+    F[3 * a + 1] = S01 * c[a][0] + S11 * c[a][1] + S12 * c[a][2];
+    F[3 * a + 2] = S02 * c[a][0] + S12 * c[a][1] + S22 * c[a][2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) F[k] = -(F[3 + k] + F[6 + k] + F[9 + k]);
+  T en = T(0);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) en += U[i] * F[i];
+  return en;
+}
+
+template <typename T>
+__device__ __forceinline__ T tri_force_free(const T *X, const T *U, T lam, T mu, T es, T *F) {
+  const T e1x = X[2] - X[0], e1y = X[3] - X[1];
+  const T e2x = X[4] - X[0], e2y = X[5] - X[1];
+  // c_1 = (e2y, -e2x), c_2 = (-e1y, e1x)
+  const T det = e1x * e2y - e1y * e2x;
+  const T s = es * recip<T>(T(2) * fabs(det));
+  const T dx1 = U[2] - U[0], dy1 = U[3] - U[1], dx2 = U[4] - U[0], dy2 = U[5] - U[1];
+  const T H00 = dx1 * e2y - dx2 * e1y, H01 = -dx1 * e2x + dx2 * e1x;
+  const T H10 = dy1 * e2y - dy2 * e1y, H11 = -dy1 * e2x + dy2 * e1x;
+  const T ls = lam * s, ms = mu * s;
+  const T tr = ls * (H00 + H11);
+  const T S00 = tr + T(2) * ms * H00;
+  const T S11 = tr + T(2) * ms * H11;
+  const T S01 = ms * (H01 + H10);
+  F[2] = S00 * e2y - S01 * e2x;
+  F[3] = S01 * e2y - S11 * e2x;
+  F[4] = -S00 * e1y + S01 * e1x;
+  F[5] = -S01 * e1y + S11 * e1x;
+  F[0] = -(F[2] + F[4]);
+  F[1] = -(F[3] + F[5]);
+  T en = T(0);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) en += U[i] * F[i];
+  return en;
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T elem_force_free(const T *X, const T *U, T lam, T mu, T es, T *F) {
+  if constexpr (DIM == 3)
+    return tet_force_free<T>(X, U, lam, mu, es, F);
+  else
+    return tri_force_free<T>(X, U, lam, mu, es, F);
+}
+
 template <typename T, int DIM>
 __device__ __forceinline__ T elem_force(const T *X, const T *U, T lam, T mu, T es, unsigned fix,
                                         T *F) {

@@ -246,16 +246,21 @@ int ensure_diag(femgpu_ctx *ctx) {
   if (!ctx->dirty_diag) return FEMGPU_OK;
   const size_t nd = size_t(ctx->nn) * ctx->dim;
   double *diag, *inv, *part;
-  float *inv32;
+  float *inv32, *diag32;
   if ((st = dbuf(ctx, "diag64", nd, &diag)) || (st = dbuf(ctx, "invD64", nd, &inv)) ||
-      (st = dbuf(ctx, "invD32", nd, &inv32)) ||
+      (st = dbuf(ctx, "invD32", nd, &inv32)) || (st = dbuf(ctx, "diag32", nd, &diag32)) ||
       (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim, &part)))
     return st;
   femgpu::launch_fill<double>(diag, int64_t(nd), 0.0, ctx->stream);
   femgpu::launch_diag<double>(ctx->dm, diag, part, ctx->stream);
   femgpu::launch_inv_diag<double>(ctx->dm, diag, 1.0, 0.0, nullptr, inv, ctx->stream);
   femgpu::launch_convert<double, float>(inv, inv32, int64_t(nd), ctx->stream);
+  femgpu::launch_convert<double, float>(diag, diag32, int64_t(nd), ctx->stream);
   HIPCHK(hipGetLastError());
+  // the patch kernels read the assembled diagonal for the rows of
+  // constrained DOFs
+  ctx->dm.diag64 = diag;
+  ctx->dm.diag32 = diag32;
   ctx->dirty_diag = false;
   return FEMGPU_OK;
 }
@@ -603,9 +608,12 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   drop_graphs(ctx);
   ctx->have_mesh = false;
   const int npe = dim + 1;
-  int emax = ctx->emax;
-  if (emax <= 0) emax = dim == 3 ? 512 : 512;
-  std::string msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, ctx->plan);
+  // one workgroup thread per patch node and at most EPT = 3 elements per
+  // thread: patches of <= 3 * threads elements and <= threads nodes
+  const int threads = ctx->threads_req ? ctx->threads_req : 1024;
+  int emax = ctx->emax > 0 ? ctx->emax : 3 * threads;
+  if (emax > 3 * threads) return fail(ctx, FEMGPU_E_ARG, "elements per patch above 3 x patch threads");
+  std::string msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, threads, true, ctx->plan);
   if (!msg.empty()) return fail(ctx, FEMGPU_E_BAD_MESH, msg);
   // degenerate elements
   for (int64_t e = 0; e < n_elems; ++e) {
@@ -651,37 +659,34 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   dm.dim = dim;
   dm.np = int(P.patches.size());
   dm.emax = P.emax_used;
+  dm.ent = (P.ent_used + 3) & ~3;
   dm.pn = P.pn_used;
   dm.nn = P.nn;
   dm.ne = P.ne;
   dm.n_owned = P.n_owned;
   dm.n_iface = P.n_iface;
   dm.n_active = P.n_owned + P.n_iface;
-  dm.patch_threads = ctx->threads_req ? ctx->threads_req : 256;
-  // one workgroup thread handles at most two elements of its patch
-  if (dm.emax > 2 * dm.patch_threads) {
-    if (ctx->threads_req)
-      return fail(ctx, FEMGPU_E_ARG, "elements per patch exceed twice the patch threads");
-    dm.patch_threads = 512;
-  }
-  if (dm.emax > 1024) return fail(ctx, FEMGPU_E_ARG, "elements per patch above 1024");
+  dm.patch_threads = threads;
+  if (dm.pn > threads || dm.emax > 3 * threads)
+    return fail(ctx, FEMGPU_E_ARG, "internal error: patch larger than the workgroup");
   int st;
   femgpu::PatchInfo *pp;
-  uint32_t *lid;
+  uint32_t *rec;
   int32_t *shg, *shs, *ss, *ei;
-  uint16_t *ne_, *inc;
-  if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "elem_lid", P.elem_lid, &lid)) ||
+  uint16_t *ne_;
+  std::vector<uint32_t> recs;
+  femgpu::pack_records(P, recs);
+  if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "erec", recs, &rec)) ||
       (st = upload(ctx, "sh_gid", P.sh_gid, &shg)) || (st = upload(ctx, "sh_slot", P.sh_slot, &shs)) ||
-      (st = upload(ctx, "node_end", P.node_end, &ne_)) || (st = upload(ctx, "epos", P.epos, &inc)) ||
+      (st = upload(ctx, "node_end", P.node_end, &ne_)) ||
       (st = upload(ctx, "slot_start", P.slot_start, &ss)) ||
       (st = upload(ctx, "elems_int", P.elems_int, &ei)))
     return st;
   dm.patches = pp;
-  dm.elem_lid = lid;
+  dm.erec = rec;
   dm.sh_gid = shg;
   dm.sh_slot = shs;
   dm.node_end = ne_;
-  dm.epos = inc;
   dm.slot_start = ss;
   dm.elems_int = ei;
   std::vector<double> c64(size_t(n_nodes) * dim);
@@ -707,14 +712,14 @@ int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads) {
   if (!ctx) return FEMGPU_E_ARG;
   if (elems_per_patch != 0 && (elems_per_patch < 16 || elems_per_patch > femgpu::EMAX_LIMIT))
     return fail(ctx, FEMGPU_E_ARG, "elems_per_patch out of range");
-  if (patch_threads != 0 && patch_threads != 256 && patch_threads != 512)
-    return fail(ctx, FEMGPU_E_ARG, "patch_threads must be 256 or 512");
-  if (patch_threads && ctx->have_mesh && ctx->dm.emax > 2 * patch_threads)
-    return fail(ctx, FEMGPU_E_ARG, "current mesh has patches larger than 2 x patch_threads");
+  if (patch_threads != 0 && patch_threads != 256 && patch_threads != 512 && patch_threads != 1024)
+    return fail(ctx, FEMGPU_E_ARG, "patch_threads must be 256, 512 or 1024");
+  const int threads = patch_threads ? patch_threads : 1024;
+  if (elems_per_patch > 3 * threads)
+    return fail(ctx, FEMGPU_E_ARG, "elems_per_patch above 3 x patch_threads");
+  // takes effect at the next femgpu_set_mesh (the plan depends on both)
   ctx->emax = elems_per_patch;
   ctx->threads_req = patch_threads;
-  if (patch_threads) ctx->dm.patch_threads = patch_threads;
-  drop_graphs(ctx);
   return FEMGPU_OK;
 }
 
@@ -823,7 +828,7 @@ int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y) {
   int st = need_mesh(ctx);
   if (st) return st;
   if (!x || !y) return FEMGPU_E_ARG;
-  if ((st = ensure_material(ctx)) || (st = ensure_fix(ctx))) return st;
+  if ((st = ensure_diag(ctx))) return st;
   const size_t nd = size_t(ctx->nn) * ctx->dim;
   std::vector<double> xi = to_internal(ctx, x), yi(nd, 0.0);
   double *dx, *dy, *part;
@@ -855,7 +860,7 @@ int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y) {
 int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps) {
   int st = need_mesh(ctx);
   if (st) return st;
-  if ((st = ensure_material(ctx)) || (st = ensure_fix(ctx))) return st;
+  if ((st = ensure_diag(ctx))) return st;
   const size_t nd = size_t(ctx->nn) * ctx->dim;
   if (precision == FEMGPU_FP64) {
     double *dx, *dy, *part;

@@ -13,16 +13,16 @@ namespace femgpu {
 struct DevMesh {
   int dim = 0;
   int np = 0;              // patches
-  int emax = 0;            // LDS stride (elements per patch bound)
+  int emax = 0;            // elements per patch bound
+  int ent = 0;             // LDS entry capacity (longest padded entry list, multiple of 4)
   int pn = 0;              // LDS node capacity
-  int patch_threads = 256; // threads per patch workgroup (256 or 512)
+  int patch_threads = 1024; // threads per patch workgroup (256, 512 or 1024)
   int64_t nn = 0, ne = 0, n_owned = 0, n_iface = 0, n_active = 0;
   const PatchInfo *patches = nullptr;
-  const uint32_t *elem_lid = nullptr;
+  const uint32_t *erec = nullptr;      // 3 words per element (pack_records)
   const int32_t *sh_gid = nullptr;
   const int32_t *sh_slot = nullptr;
   const uint16_t *node_end = nullptr;
-  const uint16_t *epos = nullptr;
   const int32_t *slot_start = nullptr;
   const uint8_t *fixmask = nullptr;   // per internal node: bit c = component c fixed
   const int32_t *elems_int = nullptr; // ne x npe internal node ids
@@ -30,6 +30,8 @@ struct DevMesh {
   const float *coords32 = nullptr;
   const double *escale64 = nullptr;   // per internal element or null
   const float *escale32 = nullptr;
+  const double *diag64 = nullptr;     // assembled diag(K), set before any matvec
+  const float *diag32 = nullptr;
   double lam = 0, mu = 0;             // Lame constants (2D: plane stress)
 };
 
@@ -64,7 +66,7 @@ struct OpShift {
   const T *m = nullptr;   // per DOF, internal order
 };
 
-int lds_bytes(int dim, int elem_bytes, int pn, int emax);
+int lds_bytes(int dim, int elem_bytes, int pn, int ent);
 // Per-kernel launch attributes (large dynamic LDS); call once per device
 // before any launch or capture.
 hipError_t prepare_kernels();

@@ -37,20 +37,19 @@ constexpr int NT = 256;          // threads per block for every kernel
 constexpr int NODE_GRID_MAX = 2048;
 
 struct LdsLayout {
-  int off_c, off_x, off_res, off_end, off_fix, off_red, total;
+  int off_c, off_x, off_res, off_red, total;
 };
 
-__host__ __device__ inline int al16(int v) { return (v + 15) & ~15; }
+__host__ __device__ inline int al16(int v) { return (v + 31) & ~31; }  // 32-byte aligned regions
 
-__host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int emax) {
-  const int npe = dim + 1, nde = dim * npe;
+// Patch kernel LDS: node coordinates and vector (pn x dim), one component of
+// the node-sorted element entries (ent, padded runs), reduction scratch.
+__host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ent) {
   LdsLayout L;
   L.off_c = 0;
   L.off_x = al16(pn * dim * tb);
   L.off_res = L.off_x + al16(pn * dim * tb);
-  L.off_end = L.off_res + al16(emax * nde * tb);
-  L.off_fix = L.off_end + al16(pn * 2);
-  L.off_red = L.off_fix + al16(pn);
+  L.off_red = L.off_res + al16(ent * tb);
   L.total = L.off_red + 16 * 8;
   return L;
 }
@@ -65,17 +64,17 @@ using namespace elem;
 template <typename T>
 struct PatchArgs {
   const PatchInfo *patches;
-  const uint32_t *elem_lid;
+  const uint32_t *erec;        // 3 words per element (pack_records)
   const int32_t *sh_gid;
   const int32_t *sh_slot;
   const uint16_t *node_end;
-  const uint16_t *epos;
   const uint8_t *fixmask;
   const int32_t *slot_start;
   const T *coords;
   const T *escale;
+  const T *diagK;     // assembled diagonal (rows of constrained DOFs), MODE 0/1
   T lam, mu;
-  int np, p8, emax, pn;
+  int np, p8, ent, pn;
   int64_t n_owned;
   const T *x;   // MODE 0: x; MODE 1: z
   T *p;         // MODE 1: search direction (old in, new out for owned nodes)
@@ -89,21 +88,60 @@ struct PatchArgs {
   int parity;         // MODE 1: j & 1
 };
 
+// Four consecutive values of T as one 16-byte (float) or 32-byte (double)
+// LDS access.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+  __device__ static type zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+  __device__ static float get(const type &v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+  }
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+  __device__ static type zero() { return make_double4(0.0, 0.0, 0.0, 0.0); }
+  __device__ static double get(const type &v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+  }
+};
+
+constexpr int EPT = 3;  // elements per thread: emax <= EPT * PT, patch nodes <= PT
+
+// Decode a 12-byte element record (pack_records): local node ids and the
+// positions of the element's corner entries in the patch's node-sorted list.
+__device__ __forceinline__ void decode_rec(uint3 r, int *l, int *q) {
+  l[0] = int(r.x & 1023u);
+  l[1] = int((r.x >> 10) & 1023u);
+  l[2] = int((r.x >> 20) & 1023u);
+  l[3] = int((r.x >> 30) | ((r.y & 255u) << 2));
+  q[0] = int((r.y >> 8) & 16383u);
+  q[1] = int((r.y >> 22) | ((r.z & 15u) << 10));
+  q[2] = int((r.z >> 4) & 16383u);
+  q[3] = int(r.z >> 18);
+}
+
+// One workgroup per patch, one patch node per thread (phases A and C), up to
+// EPT elements per thread (phase B).  The element forces stay in registers
+// and are assembled one component at a time through a single-component LDS
+// entry list, which keeps the LDS per element at npe words.
 template <typename T, int DIM, int MODE, int PT>
 __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   constexpr int NPE = DIM + 1, NDE = DIM * NPE;
+  constexpr unsigned FMASK = (1u << DIM) - 1u;
   extern __shared__ __align__(16) unsigned char lds[];
   const int b = blockIdx.x;
   // XCD-aware: consecutive patches (neighbours on the Morton curve, sharing
   // interface nodes) go to the same XCD and its L2.
   const int pid = (b & 7) * a.p8 + (b >> 3);
   if (pid >= a.np) return;
-  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.emax);
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.ent);
   T *s_c = reinterpret_cast<T *>(lds + L.off_c);
-  T *s_x = reinterpret_cast<T *>(lds + L.off_x);
-  T *s_res = reinterpret_cast<T *>(lds + L.off_res);   // node-sorted entries x DIM
-  uint16_t *s_end = reinterpret_cast<uint16_t *>(lds + L.off_end);
-  uint8_t *s_fix = reinterpret_cast<uint8_t *>(lds + L.off_fix);
+  T *s_x = reinterpret_cast<T *>(lds + L.off_x);      // constrained components zeroed
+  T *s_res = reinterpret_cast<T *>(lds + L.off_res);  // one component of the entries
   double *s_red = reinterpret_cast<double *>(lds + L.off_red);
 
   T beta = T(0);
@@ -152,127 +190,165 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   const T cK = MODE == 1 ? a.sh.cK : T(1);
   const T cM = MODE == 1 ? a.sh.cM : T(0);
 
-  // Prefetch this thread's element ids and entry positions (phase B) so
-  // their latency overlaps the node staging.
-  constexpr int EPT = 2;  // elements per thread (emax <= 2*PT)
-  uint32_t lid_pf[EPT];
-  uint2 pos_pf[EPT];
+  // Element records of this thread, loaded first so their latency overlaps
+  // the node staging.
+  uint3 rec[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * PT;
-    if (i < P.n_elem) {
-      lid_pf[e] = a.elem_lid[P.elem_start + i];
-      pos_pf[e] = reinterpret_cast<const uint2 *>(a.epos)[P.elem_start + i];
-    }
+    rec[e] = i < P.n_elem ? reinterpret_cast<const uint3 *>(a.erec)[P.elem_start + i]
+                          : make_uint3(0u, 0u, 0u);
   }
 
-  // Phase A: stage the patch's nodes in LDS.
-  for (int k = threadIdx.x; k < nloc; k += PT) {
-    const bool own = k < P.n_own;
-    const int64_t gid = own ? int64_t(P.own_start + k) : int64_t(a.sh_gid[P.sh_start + k - P.n_own]);
+  // The entry list starts zeroed: the padding entries of the node runs
+  // (never written by an element) must read as zero.
+  {
+    using V4 = typename Vec4<T>::type;
+    V4 *r4 = reinterpret_cast<V4 *>(s_res);
+    const V4 z = Vec4<T>::zero();
+    for (int i = threadIdx.x; i < (a.ent >> 2); i += PT) r4[i] = z;
+  }
+
+  // Phase A: this thread's patch node.  Constrained components enter the
+  // element products as zero (their couplings are dropped); their rows are
+  // set from the assembled diagonal below.
+  const int k = threadIdx.x;
+  const bool has_node = k < nloc;
+  const bool own = k < P.n_own;
+  int64_t gid = 0;
+  int64_t slot = 0;
+  int b0 = 0, b1 = 0;
+  unsigned fm = 0;
+  T xin[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) xin[c] = T(0);
+  if (has_node) {
+    gid = own ? int64_t(P.own_start + k) : int64_t(a.sh_gid[P.sh_start + k - P.n_own]);
+    if (!own) slot = a.sh_slot[P.sh_start + k - P.n_own];
+    b1 = int(a.node_end[P.pnode_start + k]);
+    b0 = k ? int(a.node_end[P.pnode_start + k - 1]) : 0;
 #pragma unroll
     for (int c = 0; c < DIM; ++c) s_c[k * DIM + c] = a.coords[gid * DIM + c];
+    if constexpr (MODE != 2) fm = a.fixmask[gid] & FMASK;
     if constexpr (MODE == 0) {
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) s_x[k * DIM + c] = a.x[gid * DIM + c];
+      for (int c = 0; c < DIM; ++c) xin[c] = a.x[gid * DIM + c];
     } else if constexpr (MODE == 1) {
-      T pn[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        pn[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
-        s_x[k * DIM + c] = pn[c];
-      }
+      for (int c = 0; c < DIM; ++c) xin[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
       if (own) {
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = pn[c];
-      }
-      if (a.sh.m) {
-        // mass term p'M p: owned nodes here, interface nodes by their first patch
-        const bool first = own || a.sh_slot[P.sh_start + k - P.n_own] ==
-                                      a.slot_start[gid - a.n_owned];
-        if (first) {
-#pragma unroll
-          for (int c = 0; c < DIM; ++c)
-            acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * double(pn[c]) * double(pn[c]);
-        }
+        for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
       }
     }
-    if constexpr (MODE != 2) s_fix[k] = a.fixmask[gid];
-    s_end[k] = a.node_end[P.pnode_start + k];
+    if constexpr (MODE != 2) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) s_x[k * DIM + c] = ((fm >> c) & 1u) ? T(0) : xin[c];
+    }
+  }
+  // interface nodes: does this patch hold the node's first slot?  (The
+  // first patch writes the constrained rows and the node's mass and
+  // constrained-diagonal terms of p'(cK K + cM M) p.)
+  bool first = own;
+  if constexpr (MODE == 1) {
+    if (has_node && !own && (a.sh.m || fm)) first = slot == a.slot_start[gid - a.n_owned];
+    if (has_node && first) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const double pv = double(xin[c]);
+        if (a.sh.m) acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * pv * pv;
+        if ((fm >> c) & 1u) acc_en += double(cK) * double(a.diagK[gid * DIM + c]) * pv * pv;
+      }
+    }
+  } else if constexpr (MODE == 0) {
+    if (has_node && !own && fm) first = slot == a.slot_start[gid - a.n_owned];
   }
   __syncthreads();
 
-  // Phase B: one element per thread, nodal forces written to their
-  // node-sorted entry positions.
+  // Phase B: element forces in registers.
+  T F[EPT][NDE];
+  int q[EPT][4];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * PT;
-    if (i >= P.n_elem) break;
-    const uint32_t lid = lid_pf[e];
+    int l[4];
+    decode_rec(rec[e], l, q[e]);
+    if (i >= P.n_elem) continue;
     T X[NDE], U[NDE];
-    unsigned fix = 0;
-#pragma unroll
-    for (int n = 0; n < NPE; ++n) {
-      const int l = int((lid >> (8 * n)) & 255u);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        X[n * DIM + c] = s_c[l * DIM + c];
-        if constexpr (MODE != 2) U[n * DIM + c] = s_x[l * DIM + c];
-      }
-      if constexpr (MODE != 2) fix |= unsigned(s_fix[l] & ((1 << DIM) - 1)) << (n * DIM);
-    }
-    const T es = a.escale ? a.escale[P.elem_start + i] : T(1);
-    T F[NDE];
-    if constexpr (MODE == 2) {
-      elem_diag<T, DIM>(X, a.lam, a.mu, es, F);
-    } else {
-      const T en = elem_force<T, DIM>(X, U, a.lam, a.mu, es, fix, F);
-      if constexpr (MODE == 1) acc_en += double(cK) * double(en);
-    }
-    const unsigned pos[4] = {pos_pf[e].x & 0xffffu, pos_pf[e].x >> 16, pos_pf[e].y & 0xffffu,
-                             pos_pf[e].y >> 16};
 #pragma unroll
     for (int n = 0; n < NPE; ++n)
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) s_res[pos[n] * DIM + c] = F[n * DIM + c];
-  }
-  __syncthreads();
-
-  // Phase C: each node sums its contiguous run of entries in order; owned
-  // nodes are final, interface nodes go to their partial slot.
-  for (int k = threadIdx.x; k < nloc; k += PT) {
-    const int b0 = k ? int(s_end[k - 1]) : 0, b1 = int(s_end[k]);
-    T acc[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] = T(0);
-    int j = b0;
-    for (; j + 4 <= b1; j += 4) {
-      T v[4][DIM];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) v[u][c] = s_res[(j + u) * DIM + c];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) acc[c] += v[u][c];
+      for (int c = 0; c < DIM; ++c) {
+        X[n * DIM + c] = s_c[l[n] * DIM + c];
+        if constexpr (MODE != 2) U[n * DIM + c] = s_x[l[n] * DIM + c];
+      }
+    const T es = a.escale ? a.escale[P.elem_start + i] : T(1);
+    if constexpr (MODE == 2) {
+      elem_diag<T, DIM>(X, a.lam, a.mu, es, F[e]);
+    } else {
+      const T en = elem_force_free<T, DIM>(X, U, a.lam, a.mu, es, F[e]);
+      if constexpr (MODE == 1) acc_en += double(cK) * double(en);
     }
-    for (; j < b1; ++j)
+  }
+
+  // Phase C, one component at a time: entries to their node-sorted
+  // positions, then each node sums its contiguous run in order.
+  T acc[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) acc[c] += s_res[j * DIM + c];
-    if (k < P.n_own) {
-      const int64_t gid = P.own_start + k;
+  for (int c = 0; c < DIM; ++c) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e)
+      if (threadIdx.x + e * PT < P.n_elem)
+#pragma unroll
+        for (int n = 0; n < NPE; ++n) s_res[q[e][n]] = F[e][n * DIM + c];
+    __syncthreads();
+    // runs start and end on multiples of 4 entries: 16-byte reads, two in
+    // flight per step
+    using V4 = typename Vec4<T>::type;
+    const V4 *r4 = reinterpret_cast<const V4 *>(s_res);
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    int j = b0 >> 2;
+    const int j1 = b1 >> 2;
+    for (; j + 2 <= j1; j += 2) {
+      const V4 u = r4[j], v = r4[j + 1];
+      s0 += Vec4<T>::get(u, 0) + Vec4<T>::get(v, 0);
+      s1 += Vec4<T>::get(u, 1) + Vec4<T>::get(v, 1);
+      s2 += Vec4<T>::get(u, 2) + Vec4<T>::get(v, 2);
+      s3 += Vec4<T>::get(u, 3) + Vec4<T>::get(v, 3);
+    }
+    if (j < j1) {
+      const V4 u = r4[j];
+      s0 += Vec4<T>::get(u, 0);
+      s1 += Vec4<T>::get(u, 1);
+      s2 += Vec4<T>::get(u, 2);
+      s3 += Vec4<T>::get(u, 3);
+    }
+    acc[c] = (s0 + s1) + (s2 + s3);
+    if (c + 1 < DIM) __syncthreads();
+  }
+
+  if (has_node) {
+    // rows of constrained components: the assembled diagonal only
+    // (applyConstraints, cpu_fem_utils.cpp:461-476); for an interface node
+    // its first patch writes the row and the others contribute zero
+    if constexpr (MODE != 2) {
+      if (fm) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          if ((fm >> c) & 1u) acc[c] = first ? a.diagK[gid * DIM + c] * xin[c] : T(0);
+      }
+    }
+    if (own) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         T v = acc[c];
         if constexpr (MODE == 1) {
           v = cK * v;
-          if (a.sh.m) v += cM * a.sh.m[gid * DIM + c] * s_x[k * DIM + c];
+          if (a.sh.m) v += cM * a.sh.m[gid * DIM + c] * xin[c];
         }
         a.y[gid * DIM + c] = v;
       }
     } else {
-      const int64_t slot = a.sh_slot[P.sh_start + k - P.n_own];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = MODE == 1 ? cK * acc[c] : acc[c];
     }
@@ -781,25 +857,26 @@ template <typename T>
 PatchArgs<T> patch_args(const DevMesh &dm) {
   PatchArgs<T> a{};
   a.patches = dm.patches;
-  a.elem_lid = dm.elem_lid;
+  a.erec = dm.erec;
   a.sh_gid = dm.sh_gid;
   a.sh_slot = dm.sh_slot;
   a.node_end = dm.node_end;
-  a.epos = dm.epos;
   a.fixmask = dm.fixmask;
   a.slot_start = dm.slot_start;
   if constexpr (sizeof(T) == 8) {
     a.coords = reinterpret_cast<const T *>(dm.coords64);
     a.escale = reinterpret_cast<const T *>(dm.escale64);
+    a.diagK = reinterpret_cast<const T *>(dm.diag64);
   } else {
     a.coords = reinterpret_cast<const T *>(dm.coords32);
     a.escale = reinterpret_cast<const T *>(dm.escale32);
+    a.diagK = reinterpret_cast<const T *>(dm.diag32);
   }
   a.lam = T(dm.lam);
   a.mu = T(dm.mu);
   a.np = dm.np;
   a.p8 = (dm.np + 7) / 8;
-  a.emax = dm.emax;
+  a.ent = dm.ent;
   a.pn = dm.pn;
   a.n_owned = dm.n_owned;
   return a;
@@ -813,14 +890,16 @@ hipError_t patch_attr() {
 
 template <typename T, int DIM, int MODE, int PT>
 void launch_patch_pt(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
-  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.emax);
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.ent);
   const int grid = 8 * a.p8;
   hipLaunchKernelGGL((k_ebe_patch<T, DIM, MODE, PT>), dim3(grid), dim3(PT), L.total, s, a);
 }
 
 template <typename T, int DIM, int MODE>
 void launch_patch(const DevMesh &dm, const PatchArgs<T> &a, hipStream_t s) {
-  if (dm.patch_threads == 512)
+  if (dm.patch_threads == 1024)
+    launch_patch_pt<T, DIM, MODE, 1024>(dm, a, s);
+  else if (dm.patch_threads == 512)
     launch_patch_pt<T, DIM, MODE, 512>(dm, a, s);
   else
     launch_patch_pt<T, DIM, MODE, 256>(dm, a, s);
@@ -835,8 +914,9 @@ hipError_t prepare_kernels() {
 #define FEMGPU_ATTR(T, D, M, P) \
   if ((r = patch_attr<T, D, M, P>()) != hipSuccess) e = r;
 #define FEMGPU_ATTR_TD(T, D)                                                   \
-  FEMGPU_ATTR(T, D, 0, 256) FEMGPU_ATTR(T, D, 1, 256) FEMGPU_ATTR(T, D, 2, 256) \
-  FEMGPU_ATTR(T, D, 0, 512) FEMGPU_ATTR(T, D, 1, 512) FEMGPU_ATTR(T, D, 2, 512)
+  FEMGPU_ATTR(T, D, 0, 256) FEMGPU_ATTR(T, D, 1, 256) FEMGPU_ATTR(T, D, 2, 256)     \
+  FEMGPU_ATTR(T, D, 0, 512) FEMGPU_ATTR(T, D, 1, 512) FEMGPU_ATTR(T, D, 2, 512)     \
+  FEMGPU_ATTR(T, D, 0, 1024) FEMGPU_ATTR(T, D, 1, 1024) FEMGPU_ATTR(T, D, 2, 1024)
   FEMGPU_ATTR_TD(double, 3)
   FEMGPU_ATTR_TD(double, 2)
   FEMGPU_ATTR_TD(float, 3)
@@ -846,8 +926,8 @@ hipError_t prepare_kernels() {
   return e;
 }
 
-int lds_bytes(int dim, int elem_bytes, int pn, int emax) {
-  return lds_layout(dim, elem_bytes, pn, emax).total;
+int lds_bytes(int dim, int elem_bytes, int pn, int ent) {
+  return lds_layout(dim, elem_bytes, pn, ent).total;
 }
 
 template <typename T>

@@ -40,12 +40,14 @@ uint64_t spread2(uint64_t v) {  // 31 bits -> every second bit
 }  // namespace
 
 std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
-                       const int32_t *elems, int emax, MeshPlan &plan) {
+                       const int32_t *elems, int emax, int pn_max, bool pad_runs,
+                       MeshPlan &plan) {
   if (dim != 2 && dim != 3) return "dim must be 2 or 3";
   if (nn <= 0 || ne <= 0) return "empty mesh";
   if (nn >= (int64_t(1) << 31) / 3 || ne >= (int64_t(1) << 31) / 12)
     return "mesh too large for int32 DOF indexing";
   if (emax < 16 || emax > EMAX_LIMIT) return "elements per patch out of range";
+  if (pn_max < 16 || pn_max > PN_LIMIT) return "nodes per patch out of range";
   const int npe = dim + 1;
   plan = MeshPlan();
   plan.dim = dim;
@@ -54,6 +56,7 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
   plan.nn = nn;
   plan.ne = ne;
   plan.emax = emax;
+  plan.padded = pad_runs;
 
   for (int64_t e = 0; e < ne; ++e) {
     for (int a = 0; a < npe; ++a) {
@@ -107,7 +110,7 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
       const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
       int add = 0;
       for (int a = 0; a < npe; ++a) add += stamp[en[a]] != pid;
-      if (cur_elems > 0 && (cur_elems + 1 > emax || cur_nodes + add > PN_MAX)) {
+      if (cur_elems > 0 && (cur_elems + 1 > emax || cur_nodes + add > pn_max)) {
         ++pid;
         pstart.push_back(int32_t(i));
         cur_nodes = 0;
@@ -171,7 +174,7 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
 
   // 6. Per-patch local numbering, packed ids and node-sorted entry positions.
   plan.patches.resize(np);
-  plan.elem_lid.assign(ne, 0);
+  plan.elem_lid.assign(size_t(ne) * 4, 0);
   plan.epos.assign(size_t(ne) * 4, 0);
   plan.elems_int.assign(size_t(ne) * npe, 0);
   std::vector<int32_t> local(nn, -1);
@@ -217,21 +220,19 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
     }
     P.n_sh = n_sh;
     const int32_t nloc = n_own + n_sh;
-    if (nloc > PN_MAX) return "internal error: patch node overflow";
+    if (nloc > pn_max) return "internal error: patch node overflow";
     plan.pn_used = std::max(plan.pn_used, int(nloc));
     plan.emax_used = std::max(plan.emax_used, int(P.n_elem));
     // packed ids + counts for the incidence lists
     cnt.assign(nloc, 0);
     for (int32_t i = pstart[p]; i < pstart[p + 1]; ++i) {
       const int32_t *en = &elems[int64_t(plan.elem_perm[i]) * npe];
-      uint32_t packed = 0;
       for (int a = 0; a < npe; ++a) {
         int32_t l = local[en[a]];
-        packed |= uint32_t(l) << (8 * a);
+        plan.elem_lid[size_t(i) * 4 + a] = uint16_t(l);
         ++cnt[l];
         plan.elems_int[size_t(i) * npe + a] = inv[en[a]];
       }
-      plan.elem_lid[i] = packed;
     }
     // end offsets, then each (element, corner) entry's position in the
     // node-sorted order (within a node: by element, then corner)
@@ -240,18 +241,33 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
     for (int32_t k = 0; k < nloc; ++k) {
       pos[k] = run;
       run += cnt[k];
+      if (pad_runs) run = (run + 3) & ~3;
       plan.node_end.push_back(uint16_t(run));
     }
-    for (int32_t i = 0; i < P.n_elem; ++i) {
-      uint32_t packed = plan.elem_lid[P.elem_start + i];
+    if (run >= 16384) return "internal error: entry list above 14-bit positions";
+    plan.ent_used = std::max(plan.ent_used, int(run));
+    for (int32_t i = 0; i < P.n_elem; ++i)
       for (int a = 0; a < npe; ++a) {
-        int32_t l = int32_t((packed >> (8 * a)) & 255u);
+        int32_t l = plan.elem_lid[size_t(P.elem_start + i) * 4 + a];
         plan.epos[size_t(P.elem_start + i) * 4 + a] = uint16_t(pos[l]++);
       }
-    }
     for (int32_t v : loc_nodes) local[v] = -1;
   }
   return "";
+}
+
+void pack_records(const MeshPlan &plan, std::vector<uint32_t> &rec) {
+  rec.assign(size_t(plan.ne) * 3, 0u);
+  for (int64_t e = 0; e < plan.ne; ++e) {
+    uint32_t l[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
+    for (int a = 0; a < plan.npe; ++a) {
+      l[a] = plan.elem_lid[size_t(e) * 4 + a];
+      q[a] = plan.epos[size_t(e) * 4 + a];
+    }
+    rec[size_t(e) * 3 + 0] = l[0] | l[1] << 10 | l[2] << 20 | (l[3] & 3u) << 30;
+    rec[size_t(e) * 3 + 1] = l[3] >> 2 | q[0] << 8 | (q[1] & 1023u) << 22;
+    rec[size_t(e) * 3 + 2] = q[1] >> 10 | q[2] << 4 | q[3] << 18;
+  }
 }
 
 }  // namespace femgpu

@@ -3,7 +3,7 @@
 // mesh plan built by build_plan: every index the kernels form is checked
 // against its buffer, every sorted entry position is written exactly once,
 // and the plan-based assembly of random element forces must equal direct
-// assembly.  Usage: plan_check <dim> <n> <elems_per_patch>
+// assembly.  Usage: plan_check <dim> <n> <elems_per_patch> [nodes_per_patch]
 #include "femgpu_internal.h"
 #include <cstdio>
 #include <cstdlib>
@@ -14,6 +14,7 @@ using namespace femgpu;
 #define CHK(c) do { if (!(c)) { printf("FAIL %s line %d\n", #c, __LINE__); exit(1);} } while(0)
 int main(int argc, char** argv) {
   int dim = atoi(argv[1]), n = atoi(argv[2]), emax = atoi(argv[3]);
+  int pn_max = argc > 4 ? atoi(argv[4]) : 1024;
   std::vector<double> nodes; std::vector<int32_t> elems;
   if (dim == 3) {
     int N = n+1;
@@ -26,7 +27,8 @@ int main(int argc, char** argv) {
     for (int j=0;j<ny;++j) for(int i=0;i<nx;++i){int a=i+N*j; elems.insert(elems.end(),{a,a+1,a+N+1}); elems.insert(elems.end(),{a,a+N+1,a+N});}
   }
   int npe=dim+1; long nn=nodes.size()/dim, ne=elems.size()/npe;
-  MeshPlan P; std::string m = build_plan(dim, nn, nodes.data(), ne, elems.data(), emax, P);
+  MeshPlan P; std::string m = build_plan(dim, nn, nodes.data(), ne, elems.data(), emax, pn_max, true, P);
+  std::vector<uint32_t> rec; pack_records(P, rec); CHK(rec.size() == size_t(ne) * 3);
   if(!m.empty()){printf("plan error: %s\n", m.c_str()); return 1;}
   printf("nn=%ld ne=%ld np=%zu owned=%ld iface=%ld slots=%ld pn_used=%d emax_used=%d\n", nn, ne, P.patches.size(), (long)P.n_owned, (long)P.n_iface, (long)P.n_slots, P.pn_used, P.emax_used);
   // random element "forces": f[e][a][c] ; direct assembly vs plan assembly
@@ -43,14 +45,21 @@ int main(int argc, char** argv) {
     int nloc = Q.n_own + Q.n_sh; CHK(nloc <= P.pn_used && Q.n_elem <= P.emax_used);
     std::vector<long> gidl(nloc);
     for (int k=0;k<nloc;++k){ long gid = k<Q.n_own ? Q.own_start+k : P.sh_gid.at(Q.sh_start+k-Q.n_own); CHK(gid>=0 && gid<nn); gidl[k]=gid; CHK(Q.pnode_start + k < (long)P.node_end.size()); }
-    std::vector<double> res(P.emax_used*npe*dim, NAN); std::vector<int> written(Q.n_elem*npe, 0);
-    for (int i=0;i<Q.n_elem;++i){ long ie = Q.elem_start+i; CHK(ie < ne); uint32_t lid = P.elem_lid[ie]; long ue = P.elem_perm[ie];
-      for (int a=0;a<npe;++a){ int l=(lid>>(8*a))&255; CHK(l<nloc); CHK(P.node_perm[gidl[l]] == elems[ue*npe+a]); CHK(P.elems_int[ie*npe+a]==gidl[l]);
-        int pos = P.epos[ie*4+a]; CHK(pos >= 0 && pos < Q.n_elem*npe); CHK(!written[pos]); written[pos]=1;
+    // the kernel zeroes its entry list first; padding entries stay zero
+    CHK(P.ent_used <= 16383); std::vector<double> res(P.ent_used*dim, 0.0); std::vector<int> written(P.ent_used, 0);
+    for (int i=0;i<Q.n_elem;++i){ long ie = Q.elem_start+i; CHK(ie < ne); long ue = P.elem_perm[ie];
+      // the kernel's decoding of the 12-byte record
+      uint32_t w0 = rec[ie*3], w1 = rec[ie*3+1], w2 = rec[ie*3+2];
+      int L[4] = {int(w0 & 1023u), int((w0 >> 10) & 1023u), int((w0 >> 20) & 1023u), int((w0 >> 30) | ((w1 & 255u) << 2))};
+      int Pq[4] = {int((w1 >> 8) & 16383u), int((w1 >> 22) | ((w2 & 15u) << 10)), int((w2 >> 4) & 16383u), int(w2 >> 18)};
+      for (int a=0;a<npe;++a){ int l=L[a]; CHK(l == P.elem_lid[ie*4+a]); CHK(l<nloc); CHK(P.node_perm[gidl[l]] == elems[ue*npe+a]); CHK(P.elems_int[ie*npe+a]==gidl[l]);
+        int pos = Pq[a]; CHK(pos == P.epos[ie*4+a]); CHK(pos >= 0 && pos < P.ent_used); CHK(!written[pos]); written[pos]=1;
+        { int bb0 = l? P.node_end[Q.pnode_start+l-1]:0, bb1 = P.node_end[Q.pnode_start+l]; CHK(pos >= bb0 && pos < bb1); }
         for(int c=0;c<dim;++c) res[pos*dim+c] = fe[(ue*npe+a)*dim+c]; } }
-    for (int k=0;k<nloc;++k){ int b0 = k? P.node_end[Q.pnode_start+k-1]:0, b1=P.node_end[Q.pnode_start+k]; CHK(b0<=b1 && b1 <= Q.n_elem*npe);
+    for (int k=0;k<nloc;++k){ int b0 = k? P.node_end[Q.pnode_start+k-1]:0, b1=P.node_end[Q.pnode_start+k]; CHK(b0<=b1 && b1 <= P.ent_used); CHK(b0 % 4 == 0 && b1 % 4 == 0);
+      { int nw = 0; for (int j=b0;j<b1;++j) nw += written[j]; CHK(nw > 0 && b1 - b0 - nw < 4); }
       double acc[3]={0,0,0};
-      for (int j=b0;j<b1;++j){ for(int c=0;c<dim;++c){ double v=res[j*dim+c]; CHK(!std::isnan(v)); acc[c]+=v; } }
+      for (int j=b0;j<b1;++j){ for(int c=0;c<dim;++c){ double v=res[j*dim+c]; acc[c]+=v; } }
       if (k<Q.n_own){ long gid=Q.own_start+k; CHK(gid < P.n_owned); for(int c=0;c<dim;++c) y[gid*dim+c]=acc[c]; }
       else { long slot=P.sh_slot.at(Q.sh_start+k-Q.n_own); CHK(slot>=0 && slot<P.n_slots); CHK(!slot_written[slot]); slot_written[slot]=1; for(int c=0;c<dim;++c) part[slot*dim+c]=acc[c]; } }
   }

@@ -38,8 +38,10 @@ def test_create_without_gpu_fails_cleanly():
         F.Context(0)
 
 
-@pytest.mark.parametrize("args", [(3, 5, 512), (3, 16, 512), (3, 16, 256), (3, 12, 1024),
-                                  (2, 12, 512), (2, 64, 1024), (3, 40, 768)])
+@pytest.mark.parametrize("args", [(3, 5, 512, 256), (3, 16, 512, 256), (3, 16, 256, 256),
+                                  (3, 12, 1024, 512), (2, 12, 512, 256), (2, 64, 1024, 512),
+                                  (3, 40, 768, 512), (3, 24, 3072, 1024), (3, 40, 3072, 1024), (3, 40, 2048, 512),
+                                  (2, 96, 3072, 1024)])
 def test_plan_indexing_emulation(tmp_path, args):
     exe = tmp_path / "plan_check"
     src = os.path.join(ROOT, "tests", "cpp", "plan_check.cpp")
