@@ -193,11 +193,13 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   // Element records of this thread, loaded first so their latency overlaps
   // the node staging.
   uint3 rec[EPT];
+  T esc[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * PT;
     rec[e] = i < P.n_elem ? reinterpret_cast<const uint3 *>(a.erec)[P.elem_start + i]
                           : make_uint3(0u, 0u, 0u);
+    esc[e] = a.escale && i < P.n_elem ? a.escale[P.elem_start + i] : T(1);
   }
 
   // The entry list starts zeroed: the padding entries of the node runs
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
         X[n * DIM + c] = s_c[l[n] * DIM + c];
         if constexpr (MODE != 2) U[n * DIM + c] = s_x[l[n] * DIM + c];
       }
-    const T es = a.escale ? a.escale[P.elem_start + i] : T(1);
+    const T es = esc[e];
     if constexpr (MODE == 2) {
       elem_diag<T, DIM>(X, a.lam, a.mu, es, F[e]);
     } else {
