@@ -38,6 +38,28 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
+// Two block-wide sums with one pair of barriers; sh holds 2 * BT / 64 doubles.
+template <int BT>
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+    sh[2 * w] = a;
+    sh[2 * w + 1] = b;
+  }
+  __syncthreads();
+  double ra = 0, rb = 0;
+#pragma unroll
+  for (int i = 0; i < BT / 64; ++i) {
+    ra += sh[2 * i];
+    rb += sh[2 * i + 1];
+  }
+  a = ra;
+  b = rb;
+}
+
 // ---------------------------------------------------------------------------
 // Element kernels (registers only).  X, U: node-major (npe x dim).
 // fix: bit (dim*a + c) set when that local DOF is constrained.

@@ -50,7 +50,7 @@ __host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ent
   L.off_x = al16(pn * dim * tb);
   L.off_res = L.off_x + al16(pn * dim * tb);
   L.off_red = L.off_res + al16(ent * tb);
-  L.total = L.off_red + 16 * 8;
+  L.total = L.off_red + 32 * 8;
   return L;
 }
 
@@ -155,8 +155,7 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
       g += a.prz[i];
       q += a.prr[i];
     }
-    g = block_sum<PT>(g, s_red);
-    q = block_sum<PT>(q, s_red);
+    block_sum2<PT>(g, q, s_red);
     const int q1 = a.parity ^ 1;
     const int n_upd = sc->it[q1] + 1;
     int stop = 0;
@@ -406,7 +405,7 @@ constexpr int UPT = 512;  // threads per block of the node-parallel PCG kernels
 // per-block r'z, r'r partials for the next iteration's stop test.
 template <typename T, int DIM>
 __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
-  __shared__ double s_red[UPT / 64];
+  __shared__ double s_red[2 * UPT / 64];
   CgScalars *sc = a.sc;
   if (sc->done) return;
   double pq = 0.0;
@@ -459,8 +458,7 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
       rr += double(rv) * double(rv);
     }
   }
-  rz = block_sum<UPT>(rz, s_red);
-  rr = block_sum<UPT>(rr, s_red);
+  block_sum2<UPT>(rz, rr, s_red);
   if (threadIdx.x == 0) {
     a.prz[blockIdx.x] = rz;
     a.prr[blockIdx.x] = rr;
