@@ -66,9 +66,29 @@ def test_missing_mesh_is_an_error(tmp_path):
     assert r.returncode == 1 and "cannot open" in r.stderr
 
 
-def test_run_fem_reports_missing_prepared_mesh(tmp_path):
+def test_run_fem_reports_missing_raw_mesh(tmp_path):
     p = tmp_path / "c.json"
-    p.write_text(json.dumps({"DIM": 2, "working_path": str(tmp_path) + "/", "task": ["nope"]}))
+    p.write_text(json.dumps({"DIM": 2, "working_path": str(tmp_path) + "/", "task": ["nope"],
+                             "CudaFEMproject_path": str(tmp_path) + "/"}))
     r = subprocess.run(["python", os.path.join(ROOT, "run_fem.py"), str(p)], capture_output=True,
                        text=True)
-    assert r.returncode == 1 and "prepared mesh for nope not found" in r.stdout
+    assert r.returncode == 0 and "raw mesh nope.k doesn't exist" in r.stdout   # as run_fem.py:46-48
+
+
+def test_run_fem_parses_raw_mesh(tmp_path):
+    """No prepared mesh: run_fem.py converts raw_meshes/2D/9task_3.k and hands
+    the task to the solver (dry run here: no GPU needed)."""
+    raw = tmp_path / "proj" / "raw_meshes" / "2D"
+    raw.mkdir(parents=True)
+    shutil.copy(os.path.join(GOLDEN, "9task_3", "9task_3.k"), raw / "9task_3.k")
+    p = tmp_path / "c.json"
+    p.write_text(json.dumps({"DIM": 2, "working_path": str(tmp_path / "work") + "/",
+                             "CudaFEMproject_path": str(tmp_path / "proj") + "/",
+                             "task": ["9task_3"], "youngModulus": 2e7, "poissonRatio": 0.25}))
+    r = subprocess.run(["python", os.path.join(ROOT, "run_fem.py"), str(p), "--dry-run"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FINISHED parser for 9task_3" in r.stdout and "Nodes count = 35" in r.stdout
+    prep = tmp_path / "work" / "prepared_meshes" / "2D" / "9task_3"
+    for f in ("nodes.txt", "elements.txt", "constraints.txt", "loads.txt", "stress.txt"):
+        assert (prep / f).exists()
