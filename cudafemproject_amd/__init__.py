@@ -6,7 +6,8 @@ kernels for gfx950 behind the C-ABI of include/femgpu.h.
 """
 from ._lib import (FP32, FP64, MIXED, IMPLICIT_PHYSICAL, IMPLICIT_REFERENCE, LOADS_PHYSICAL,
                    LOADS_REFERENCE, MASS_PHYSICAL, MASS_REFERENCE, TOL_REF_GAMMA, TOL_REL_RES,
-                   WAVELET_BERLAGE, WAVELET_NONE, WAVELET_RICKER, Context, FemGpuError,
+                   RECV_ACCELERATION, RECV_DISPLACEMENT, RECV_VELOCITY, WAVELET_BERLAGE,
+                   WAVELET_NONE, WAVELET_RICKER, Context, FemGpuError,
                    load_library)
 from .problem import Problem
 

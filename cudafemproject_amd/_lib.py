@@ -28,6 +28,7 @@ LOADS_REFERENCE, LOADS_PHYSICAL = 0, 1
 MASS_REFERENCE, MASS_PHYSICAL = 0, 1
 WAVELET_NONE, WAVELET_RICKER, WAVELET_BERLAGE = 0, 1, 2
 IMPLICIT_REFERENCE, IMPLICIT_PHYSICAL = 0, 1
+RECV_DISPLACEMENT, RECV_VELOCITY, RECV_ACCELERATION = 0, 1, 2
 
 
 class SolveOpts(C.Structure):
@@ -46,7 +47,9 @@ class DynOpts(C.Structure):
                 ("wavelet_freq", C.c_double), ("wavelet_timeshift", C.c_double),
                 ("wavelet_ampl", C.c_double), ("precision", C.c_int),
                 ("mass_convention", C.c_int), ("implicit_mode", C.c_int),
-                ("pcg_eps", C.c_double), ("steps_per_graph", C.c_int)]
+                ("pcg_eps", C.c_double), ("steps_per_graph", C.c_int),
+                ("damping_alpha", C.c_double), ("damping_beta", C.c_double),
+                ("relax_reference", C.c_int), ("max_steps", C.c_int)]
 
 
 class DynStats(C.Structure):
@@ -93,6 +96,10 @@ SIGNATURES = {
     "femgpu_batch_set_constraints": (C.c_int, [_P, _L, _I, _I]),
     "femgpu_batch_solve": (C.c_int, [_P, C.c_int, _D, C.POINTER(SolveOpts), _D,
                                      C.POINTER(BatchStat), _D]),
+    "femgpu_set_receivers": (C.c_int, [_P, C.c_int, _I, C.c_int, C.c_int]),
+    "femgpu_get_traces": (C.c_int, [_P, _D, C.c_int64, C.POINTER(C.c_int)]),
+    "femgpu_write_segy": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_float,
+                                    C.c_int]),
     "femgpu_solve_rhs": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, C.POINTER(Stats)]),
     "femgpu_material_grad": (C.c_int, [_P, _D, _D, _D]),
     "femgpu_compliance_grad": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, _D, C.POINTER(Stats)]),
@@ -382,6 +389,20 @@ class Context:
         us = [u[noff[i]:noff[i + 1]] for i in range(len(rhs))]
         return us, stats, ms.value
 
+    def set_receivers(self, nodes, quantity: int = RECV_DISPLACEMENT, every: int = 1):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        self._check(self.lib.femgpu_set_receivers(self.h, len(nodes), _ip(nodes) if len(nodes) else None,
+                                                  quantity, every), "set_receivers")
+        self._n_recv = len(nodes)
+
+    def traces(self) -> np.ndarray:
+        """Receiver traces of the last dyn_run: (n_samples, n_receivers, dim)."""
+        n = C.c_int(0)
+        self._check(self.lib.femgpu_get_traces(self.h, None, 0, C.byref(n)), "get_traces")
+        out = np.zeros((n.value, getattr(self, "_n_recv", 0), self.dim))
+        self._check(self.lib.femgpu_get_traces(self.h, _dp(out), out.size, C.byref(n)), "get_traces")
+        return out
+
     def stress(self, u: np.ndarray) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64)
         nv = 3 * (self.dim - 1)
@@ -398,9 +419,13 @@ class Context:
     def dyn_run(self, rho=2400.0, beta1=0.5, beta2=0.0, dt=1e-3, endtime=0.1,
                 wavelet_type=WAVELET_NONE, freq=15.0, timeshift=0.0, ampl=1.0,
                 precision=FP64, mass_convention=MASS_REFERENCE,
-                implicit_mode=IMPLICIT_REFERENCE, pcg_eps=1e-4, steps_per_graph=64):
+                implicit_mode=IMPLICIT_REFERENCE, pcg_eps=1e-4, steps_per_graph=64,
+                damping_alpha=0.0, damping_beta=0.0, relax_reference=True, max_steps=1000000,
+                raise_on_error=True):
         o = DynOpts()
         self.lib.femgpu_default_dyn_opts(C.byref(o))
+        o.damping_alpha, o.damping_beta = damping_alpha, damping_beta
+        o.relax_reference, o.max_steps = int(bool(relax_reference)), max_steps
         o.rho, o.beta1, o.beta2, o.dt, o.endtime = rho, beta1, beta2, dt, endtime
         o.wavelet_type, o.wavelet_freq, o.wavelet_timeshift, o.wavelet_ampl = (
             wavelet_type, freq, timeshift, ampl)
@@ -408,6 +433,19 @@ class Context:
         o.pcg_eps, o.steps_per_graph = pcg_eps, steps_per_graph
         u = np.zeros((self.n_nodes, self.dim))
         s = DynStats()
-        self._check(self.lib.femgpu_dyn_run(self.h, C.byref(o), _dp(u), C.byref(s)), "dyn_run")
+        st = self.lib.femgpu_dyn_run(self.h, C.byref(o), _dp(u), C.byref(s))
+        if raise_on_error:
+            self._check(st, "dyn_run")
         return u, {"status": s.status, "steps": s.steps, "pcg_iterations": s.pcg_iterations,
                    "run_ms": s.run_ms}
+
+
+def write_segy(path: str, traces: np.ndarray, sample_interval: float, trace_id: int = 0):
+    """SEG-Y file of traces (n_traces, n_samples) (femgpu_write_segy)."""
+    tr = np.ascontiguousarray(traces, dtype=np.float32)
+    if tr.ndim != 2:
+        raise ValueError("traces must be (n_traces, n_samples)")
+    st = load_library().femgpu_write_segy(path.encode(), tr.ctypes.data_as(C.POINTER(C.c_float)),
+                                          tr.shape[0], tr.shape[1], sample_interval, trace_id)
+    if st != OK:
+        raise FemGpuError(st, f"cannot write {path}")

@@ -98,6 +98,10 @@ struct femgpu_ctx {
 
   bool have_mesh = false;
   femgpu::BatchState *batch = nullptr;  // batched solve (batch.cpp)
+  std::vector<int32_t> recv;            // receiver nodes (user ids)
+  int recv_quantity = 0, recv_every = 1;
+  std::vector<double> traces;           // last run: n_samples x n_recv x dim
+  int n_samples = 0;
   MeshPlan plan;
   DevMesh dm;
   std::map<std::string, DevBuf> bufs;
@@ -649,6 +653,9 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   ctx->fp.clear();
   ctx->lnode.clear();
   ctx->lvec.clear();
+  ctx->recv.clear();
+  ctx->traces.clear();
+  ctx->n_samples = 0;
   ctx->extra_rhs.clear();
   ctx->fix_user.assign(n_nodes, 0);
   ctx->ctype.assign(n_nodes, 0);
@@ -1204,6 +1211,35 @@ int femgpu_default_dyn_opts(femgpu_dyn_opts *o) {
   o->implicit_mode = FEMGPU_IMPLICIT_REFERENCE;
   o->pcg_eps = 1e-4;
   o->steps_per_graph = 64;
+  o->damping_alpha = 0.0;
+  o->damping_beta = 0.0;
+  o->relax_reference = 1;
+  o->max_steps = 1000000;
+  return FEMGPU_OK;
+}
+
+int femgpu_set_receivers(femgpu_ctx *ctx, int n, const int32_t *nodes, int quantity, int every) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (n < 0 || (n > 0 && !nodes) || every < 1 || quantity < FEMGPU_RECV_DISPLACEMENT ||
+      quantity > FEMGPU_RECV_ACCELERATION)
+    return fail(ctx, FEMGPU_E_ARG, "bad receiver arguments");
+  for (int i = 0; i < n; ++i)
+    if (nodes[i] < 0 || nodes[i] >= ctx->nn) return fail(ctx, FEMGPU_E_ARG, "receiver node out of range");
+  ctx->recv.assign(nodes, nodes + n);
+  ctx->recv_quantity = quantity;
+  ctx->recv_every = every;
+  ctx->traces.clear();
+  ctx->n_samples = 0;
+  return FEMGPU_OK;
+}
+
+int femgpu_get_traces(femgpu_ctx *ctx, double *out, int64_t capacity, int *n_samples) {
+  if (!ctx || !n_samples) return FEMGPU_E_ARG;
+  *n_samples = ctx->n_samples;
+  if (!out) return FEMGPU_OK;
+  if (capacity < int64_t(ctx->traces.size())) return fail(ctx, FEMGPU_E_ARG, "trace buffer too small");
+  std::copy(ctx->traces.begin(), ctx->traces.end(), out);
   return FEMGPU_OK;
 }
 
@@ -1252,6 +1288,15 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   const int d = ctx->dim;
   const size_t nd = size_t(ctx->nn) * d;
   const bool expl = o.beta2 == 0.0;
+  const double alpha = o.damping_alpha, beta_d = o.damping_beta;
+  const bool damped = alpha != 0.0 || beta_d != 0.0;
+  if (!(alpha >= 0.0) || !(beta_d >= 0.0)) return fail(ctx, FEMGPU_E_ARG, "damping must be >= 0");
+  if (expl && beta_d != 0.0)
+    return fail(ctx, FEMGPU_E_ARG, "explicit dynamics needs damping_beta == 0 (femfunc.cpp:145)");
+  if (!expl && damped && o.implicit_mode != FEMGPU_IMPLICIT_PHYSICAL)
+    return fail(ctx, FEMGPU_E_UNSUPPORTED, "implicit damping needs FEMGPU_IMPLICIT_PHYSICAL");
+  const bool relax = o.endtime < 0 && !o.relax_reference;
+  const double eps_relax = std::fabs(o.endtime);
   int st;
   if ((st = ensure_diag(ctx))) return st;
   // Time grid, GN22 coefficients and wavelet values follow the reference
@@ -1260,8 +1305,11 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   // state as the reference computes them and in double for FP64 state.
   const bool f32 = sizeof(T) == sizeof(float);
   const double dtd = f32 ? double(float(o.dt)) : o.dt;
-  const int endnt = o.endtime < 0 ? 1
-                    : f32 ? int(float(o.endtime) / float(o.dt)) : int(o.endtime / o.dt);
+  int endnt;
+  if (relax)
+    endnt = o.max_steps > 0 ? o.max_steps : 1000000;
+  else
+    endnt = o.endtime < 0 ? 1 : f32 ? int(float(o.endtime) / float(o.dt)) : int(o.endtime / o.dt);
   if (endnt < 1) return fail(ctx, FEMGPU_E_ARG, "endtime/dt gives no step");
   std::vector<double> cfile(endnt + 1, 1.0), cunit(endnt + 1, 0.0);
   {
@@ -1298,7 +1346,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   }
   const std::string tg = sizeof(T) == 8 ? "d64" : "d32";
   T *dd, *dv, *da, *dy, *dpart, *dff, *dfu, *dinvM, *dml;
-  double *dcf, *dcu;
+  double *dcf, *dcu, *dmaxp, *dmax;
   int *dstep;
   if ((st = dbuf(ctx, tg + "d", nd, &dd)) || (st = dbuf(ctx, tg + "v", nd, &dv)) ||
       (st = dbuf(ctx, tg + "a", nd, &da)) || (st = dbuf(ctx, tg + "y", nd, &dy)) ||
@@ -1306,8 +1354,40 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
       (st = upload(ctx, tg + "ff", ff_t, &dff)) || (st = upload(ctx, tg + "fu", fu_t, &dfu)) ||
       (st = upload(ctx, tg + "invM", invM, &dinvM)) || (st = upload(ctx, tg + "ml", mlump, &dml)) ||
       (st = upload(ctx, "dyn_cf", cfile, &dcf)) || (st = upload(ctx, "dyn_cu", cunit, &dcu)) ||
-      (st = dbuf(ctx, "dyn_step", 1, &dstep)))
+      (st = dbuf(ctx, "dyn_step", 1, &dstep)) || (st = dbuf(ctx, "dyn_maxp", 256, &dmaxp)) ||
+      (st = dbuf(ctx, "dyn_max", 1, &dmax)))
     return st;
+  // receivers
+  const int nrec = int(ctx->recv.size());
+  const int every = ctx->recv_every;
+  const int max_samples = nrec ? endnt / every : 0;
+  int32_t *drecv = nullptr;
+  T *dtr = nullptr;
+  ctx->traces.clear();
+  ctx->n_samples = 0;
+  if (nrec) {
+    if (int64_t(max_samples) * nrec * d > (int64_t(1) << 27))
+      return fail(ctx, FEMGPU_E_ARG, "receiver traces too large: increase `every`");
+    std::vector<int32_t> ri(nrec);
+    for (int i = 0; i < nrec; ++i) ri[i] = ctx->plan.node_inv[ctx->recv[i]];
+    if ((st = upload(ctx, "recv", ri, &drecv)) ||
+        (st = dbuf(ctx, tg + "traces", size_t(std::max(max_samples, 1)) * nrec * d, &dtr)))
+      return st;
+  }
+  const T *rsrc = ctx->recv_quantity == FEMGPU_RECV_VELOCITY ? dv
+                  : ctx->recv_quantity == FEMGPU_RECV_ACCELERATION ? da : dd;
+  auto record = [&](int step_off) {
+    if (nrec && max_samples > 0)
+      femgpu::launch_record<T>(dstep, step_off, every, rsrc, drecv, nrec, d, max_samples, dtr,
+                               ctx->stream);
+  };
+  // max |v| for relaxation (device reduction, polled)
+  auto vmax = [&](double *out) -> int {
+    femgpu::launch_absmax<T>(int64_t(ctx->dm.n_active) * d, dv, dmaxp, dmax, ctx->stream);
+    HIPCHK(hipMemcpyAsync(out, dmax, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FEMGPU_OK;
+  };
   femgpu::launch_fill<T>(dd, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(dv, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(da, int64_t(nd), T(0), ctx->stream);
@@ -1328,13 +1408,17 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     C.c3 = o.beta1 * dtd;
     C.c4 = 0.5 * o.beta2 * dtd * dtd;
   }
+  C.alpha = alpha;
+  C.expl_scale = 1.0 / (1.0 + alpha * C.c3);
   femgpu_dyn_stats S{};
+  int steps = 0;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   if (expl) {
     // Steps are captured in a HIP graph of G steps when at least one full
     // graph fits the run; the remainder (or the whole run) is launched eagerly.
     const int G = o.steps_per_graph > 0 ? o.steps_per_graph : 64;
     int done = 0;
+    bool settled = false;
     if (G <= endnt) {
       hipGraphExec_t exec = nullptr;
       hipGraph_t g;
@@ -1344,12 +1428,13 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
         femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
         femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM,
                                       da, dv, nullptr, C, ctx->stream);
+        record(k);
       }
       HIPCHK(hipStreamEndCapture(ctx->stream, &g));
       const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIPCHK(ie);
-      while (done + G <= endnt) {
+      while (done + G <= endnt && !settled) {
         femgpu::launch_set_int(dstep, done + 1, ctx->stream);
         const hipError_t le = hipGraphLaunch(exec, ctx->stream);
         if (le != hipSuccess) {
@@ -1358,42 +1443,50 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
           HIPCHK(le);
         }
         done += G;
+        if (relax) {
+          double vm = 0.0;
+          if ((st = vmax(&vm))) {
+            (void)hipGraphExecDestroy(exec);
+            return st;
+          }
+          settled = vm < eps_relax;
+        }
       }
       HIPCHK(hipStreamSynchronize(ctx->stream));
       (void)hipGraphExecDestroy(exec);
     }
     femgpu::launch_set_int(dstep, done + 1, ctx->stream);
-    for (int k = 0; done < endnt; ++k, ++done) {
+    for (int k = 0; done < endnt && !settled; ++k, ++done) {
       femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
       femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
       femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM, da,
                                     dv, nullptr, C, ctx->stream);
+      record(k);
+      if (relax) {
+        double vm = 0.0;
+        if ((st = vmax(&vm))) return st;
+        if (vm < eps_relax) settled = true;
+      }
     }
     HIPCHK(hipGetLastError());
-    S.steps = endnt;
+    steps = done;
   } else {
-    // implicit: nested PCG per step on A a = F - K d.
+    // implicit: nested PCG per step on A a = F - K d (+ damping terms).
     PcgWs<T> w;
     if ((st = pcg_ws(ctx, sizeof(T) == 8 ? "c64" : "c32", w))) return st;
-    T *invA;
+    T *invA, *dt_tmp = nullptr;
     if ((st = dbuf(ctx, tg + "invA", nd, &invA))) return st;
+    if (beta_d != 0.0 && (st = dbuf(ctx, tg + "dtmp", nd, &dt_tmp))) return st;
     OpShift<T> sh;
-    const T ceff = T(C.c4);  // 0.5 beta2 dt^2
-    const T *diagK_t;
-    if (sizeof(T) == 8) {
-      diagK_t = reinterpret_cast<const T *>(ctx->bufs["diag64"].p);
-    } else {
-      float *d32;
-      if ((st = dbuf(ctx, "diag32", nd, &d32))) return st;
-      femgpu::launch_convert<double, float>(static_cast<double *>(ctx->bufs["diag64"].p), d32,
-                                            int64_t(nd), ctx->stream);
-      diagK_t = reinterpret_cast<const T *>(d32);
-    }
+    const T ceff = T(C.c4 + C.c3 * beta_d);  // beta2 dt^2/2 (+ beta1 dt beta)
+    const T cmass = T(1.0 + C.c3 * alpha);   // 1 (+ beta1 dt alpha)
+    const T *diagK_t = sizeof(T) == 8 ? reinterpret_cast<const T *>(ctx->dm.diag64)
+                                      : reinterpret_cast<const T *>(ctx->dm.diag32);
     if (o.implicit_mode == FEMGPU_IMPLICIT_PHYSICAL) {
       sh.cK = ceff;
-      sh.cM = T(1);
+      sh.cM = cmass;
       sh.m = dml;
-      femgpu::launch_inv_diag<T>(ctx->dm, diagK_t, ceff, T(1), dml, invA, ctx->stream);
+      femgpu::launch_inv_diag<T>(ctx->dm, diagK_t, ceff, cmass, dml, invA, ctx->stream);
     } else {
       // reference: operator K, preconditioner cK diag(K) + diag(M_consistent)
       std::vector<double> mc = host_mass(ctx, o.rho, false, mconv);
@@ -1407,12 +1500,19 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     CgScalars *sc;
     if ((st = dbuf(ctx, "sc_main", 1, &sc))) return st;
     int total_it = 0;
-    for (int nt = 1; nt <= endnt; ++nt) {
+    int nt = 1;
+    for (; nt <= endnt; ++nt) {
       femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, true, ctx->stream);
-      femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+      if (beta_d != 0.0) {
+        // K (d~ + beta v~)
+        femgpu::launch_xpby<T>(int64_t(nd), dd, dv, beta_d, dt_tmp, ctx->stream);
+        femgpu::launch_matvec_patch<T>(ctx->dm, dt_tmp, dy, dpart, ctx->stream);
+      } else {
+        femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+      }
       femgpu::launch_set_int(dstep, nt, ctx->stream);
       femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, 0, nullptr, da,
-                                    dv, w.r, C, ctx->stream);
+                                    dv, w.r, C, ctx->stream, dml);
       int it = 0, done = 0;
       if ((st = run_pcg<T>(ctx, w, sh, FEMGPU_TOL_REF_GAMMA, o.pcg_eps, 1000000, 8,
                            sizeof(T) == 8 ? 10 : 11, sc, &it, &done)))
@@ -1426,9 +1526,18 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
       }
       femgpu::launch_copy<T>(w.x, da, int64_t(nd), ctx->stream);
       femgpu::launch_dyn_implicit_post<T>(ctx->dm, da, dv, dd, C, ctx->stream);
+      record(0);
+      if (relax) {
+        double vm = 0.0;
+        if ((st = vmax(&vm))) return st;
+        if (vm < eps_relax) {
+          ++nt;
+          break;
+        }
+      }
     }
     HIPCHK(hipGetLastError());
-    S.steps = endnt;
+    steps = nt - 1;
     S.pcg_iterations = total_it;
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1436,6 +1545,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   S.run_ms = ms;
+  S.steps = steps;
   if (u_out) {
     std::vector<T> xt(nd);
     HIPCHK(hipMemcpyAsync(xt.data(), dd, nd * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
@@ -1443,7 +1553,23 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     std::vector<double> xi(xt.begin(), xt.end());
     from_internal(ctx, xi, u_out);
   }
+  if (nrec) {
+    const int ns = std::min(max_samples, steps / every);
+    std::vector<T> tr(size_t(std::max(ns, 0)) * nrec * d);
+    if (ns > 0) {
+      HIPCHK(hipMemcpyAsync(tr.data(), dtr, tr.size() * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->traces.assign(tr.begin(), tr.end());
+    ctx->n_samples = std::max(ns, 0);
+  }
   S.status = FEMGPU_OK;
+  if (relax && steps >= endnt) {
+    S.status = FEMGPU_E_MAXIT;
+    ctx->err = "relaxation did not reach max|v| < eps within max_steps";
+    if (stats) *stats = S;
+    return S.status;
+  }
   if (stats) *stats = S;
   return FEMGPU_OK;
 }

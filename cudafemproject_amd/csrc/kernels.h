@@ -129,6 +129,8 @@ void launch_bilinear(const DevMesh &dm, const T *u, const T *w, double factor, T
 // Dynamics.
 struct DynCoeffs {
   double dt, c1, c2, c3, c4;  // predictor / corrector coefficients
+  double alpha = 0.0;         // Rayleigh mass coefficient (damping C = alpha M + beta K)
+  double expl_scale = 1.0;    // explicit: 1 / (1 + alpha c3)
 };
 template <typename T>
 void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
@@ -138,10 +140,20 @@ template <typename T>
 void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T *f_file,
                         const T *f_unit, const double *coef_file, const double *coef_unit,
                         const int *step_base, int step_off, const T *invM, T *a, T *v,
-                        T *r_out, DynCoeffs c, hipStream_t s);
+                        T *r_out, DynCoeffs c, hipStream_t s, const T *mlump = nullptr);
 template <typename T>
 void launch_dyn_implicit_post(const DevMesh &dm, const T *a, T *v, T *d, DynCoeffs c,
                               hipStream_t s);
+// out = x + b y
+template <typename T>
+void launch_xpby(int64_t n, const T *x, const T *y, double b, T *out, hipStream_t s);
+// receiver traces (samples at steps that are multiples of `every`)
+template <typename T>
+void launch_record(const int *step_base, int step_off, int every, const T *src, const int32_t *recv,
+                   int nrec, int dim, int max_samples, T *traces, hipStream_t s);
+// *out = max |x_i| (part: 256 doubles)
+template <typename T>
+void launch_absmax(int64_t n, const T *x, double *part, double *out, hipStream_t s);
 void launch_set_int(int *p, int v, hipStream_t s);
 void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s);
 

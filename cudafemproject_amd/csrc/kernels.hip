@@ -800,7 +800,8 @@ __global__ __launch_bounds__(NT) void k_dyn_correct(int64_t n_owned, int64_t n_a
                                                     const T *f_unit, const double *coef_file,
                                                     const double *coef_unit, const int *step_base,
                                                     int step_off, const T *invM, T *a, T *v,
-                                                    T *r_out, T c3) {
+                                                    T *r_out, T c3, const T *mlump, T alpha,
+                                                    T scale) {
   const int step = *step_base + step_off;
   const T cf = T(coef_file[step]), cu = T(coef_unit[step]);
   const int64_t stride = int64_t(gridDim.x) * NT;
@@ -822,11 +823,13 @@ __global__ __launch_bounds__(NT) void k_dyn_correct(int64_t n_owned, int64_t n_a
       const int64_t o = i * DIM + c;
       const T r = cf * f_file[o] + cu * f_unit[o] - kd[c];
       if (invM) {
-        const T av = r * invM[o];
+        // explicit; with mass-proportional damping alpha M:
+        // (1 + alpha c3) M a = F - K d - alpha M v_pred
+        const T av = alpha != T(0) ? (r * invM[o] - alpha * v[o]) * scale : r * invM[o];
         a[o] = av;
         v[o] = v[o] + c3 * av;
       } else {
-        r_out[o] = r;
+        r_out[o] = alpha != T(0) ? r - alpha * mlump[o] * v[o] : r;
       }
     }
   }
@@ -839,6 +842,64 @@ __global__ __launch_bounds__(NT) void k_dyn_implicit_post(int64_t n, const T *a,
   for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
     v[i] = v[i] + c3 * a[i];
     d[i] = d[i] + c4 * a[i];
+  }
+}
+
+// out = x + b y
+template <typename T>
+__global__ __launch_bounds__(NT) void k_xpby(int64_t n, const T *x, const T *y, T b, T *out) {
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride)
+    out[i] = x[i] + b * y[i];
+}
+
+// Receiver traces: at steps that are multiples of `every`, copy the state
+// vector of the receiver nodes into sample (step / every - 1).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_record(const int *step_base, int step_off, int every,
+                                               const T *src, const int32_t *recv, int nrec,
+                                               int dim, int max_samples, T *traces) {
+  const int step = *step_base + step_off;
+  if (step % every != 0) return;
+  const int smp = step / every - 1;
+  if (smp < 0 || smp >= max_samples) return;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= nrec * dim) return;
+  const int r = i / dim, c = i - r * dim;
+  traces[(int64_t(smp) * nrec + r) * dim + c] = src[int64_t(recv[r]) * dim + c];
+}
+
+// max |x_i| in double: per-block partials, then one block.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_absmax_part(int64_t n, const T *x, double *part) {
+  __shared__ double sh[NT / 64];
+  double m = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride)
+    m = fmax(m, fabs(double(x[i])));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < NT / 64; ++w) r = fmax(r, sh[w]);
+    part[blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_absmax_fin(const double *part, int np, double *out) {
+  __shared__ double sh[NT / 64];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < np; i += NT) m = fmax(m, part[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < NT / 64; ++w) r = fmax(r, sh[w]);
+    *out = r;
   }
 }
 
@@ -1142,16 +1203,35 @@ template <typename T>
 void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T *f_file,
                         const T *f_unit, const double *coef_file, const double *coef_unit,
                         const int *step_base, int step_off, const T *invM, T *a, T *v, T *r_out,
-                        DynCoeffs c, hipStream_t s) {
+                        DynCoeffs c, hipStream_t s, const T *mlump) {
   const int g = grid_for(dm.n_active);
   if (dm.dim == 3)
     hipLaunchKernelGGL((k_dyn_correct<T, 3>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
                        dm.slot_start, y, partial, f_file, f_unit, coef_file, coef_unit, step_base,
-                       step_off, invM, a, v, r_out, T(c.c3));
+                       step_off, invM, a, v, r_out, T(c.c3), mlump, T(c.alpha), T(c.expl_scale));
   else
     hipLaunchKernelGGL((k_dyn_correct<T, 2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
                        dm.slot_start, y, partial, f_file, f_unit, coef_file, coef_unit, step_base,
-                       step_off, invM, a, v, r_out, T(c.c3));
+                       step_off, invM, a, v, r_out, T(c.c3), mlump, T(c.alpha), T(c.expl_scale));
+}
+
+template <typename T>
+void launch_xpby(int64_t n, const T *x, const T *y, double b, T *out, hipStream_t s) {
+  hipLaunchKernelGGL((k_xpby<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, x, y, T(b), out);
+}
+
+template <typename T>
+void launch_record(const int *step_base, int step_off, int every, const T *src, const int32_t *recv,
+                   int nrec, int dim, int max_samples, T *traces, hipStream_t s) {
+  const int g = (nrec * dim + NT - 1) / NT;
+  hipLaunchKernelGGL((k_record<T>), dim3(g), dim3(NT), 0, s, step_base, step_off, every, src,
+                     recv, nrec, dim, max_samples, traces);
+}
+
+template <typename T>
+void launch_absmax(int64_t n, const T *x, double *part, double *out, hipStream_t s) {
+  hipLaunchKernelGGL((k_absmax_part<T>), dim3(256), dim3(NT), 0, s, n, x, part);
+  hipLaunchKernelGGL(k_absmax_fin, dim3(1), dim3(NT), 0, s, part, 256, out);
 }
 
 template <typename T>
@@ -1190,9 +1270,14 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
                                       hipStream_t);                                            \
   template void launch_dyn_correct<T>(const DevMesh &, const T *, const T *, const T *,        \
                                       const T *, const double *, const double *, const int *,  \
-                                      int, const T *, T *, T *, T *, DynCoeffs, hipStream_t);  \
+                                      int, const T *, T *, T *, T *, DynCoeffs, hipStream_t,   \
+                                      const T *);                                              \
   template void launch_dyn_implicit_post<T>(const DevMesh &, const T *, T *, T *, DynCoeffs,   \
-                                            hipStream_t);
+                                            hipStream_t);                                      \
+  template void launch_xpby<T>(int64_t, const T *, const T *, double, T *, hipStream_t);       \
+  template void launch_record<T>(const int *, int, int, const T *, const int32_t *, int, int, \
+                                 int, T *, hipStream_t);                                       \
+  template void launch_absmax<T>(int64_t, const T *, double *, double *, hipStream_t);
 
 FEMGPU_INST(double)
 FEMGPU_INST(float)

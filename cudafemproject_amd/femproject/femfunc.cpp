@@ -1,7 +1,9 @@
 // femfunc.cpp -- see femfunc.h.
 #include "femfunc.h"
 
+#include <algorithm>
 #include <cmath>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -126,14 +128,56 @@ void gpuCalculateFEM_DYN(DEVICE_NAME device, dataKeeper &dk, bool PRINT_DEBUG_IN
   o.mass_convention = opt.mass_convention;
   o.implicit_mode = opt.implicit_mode;
   o.pcg_eps = opt.pcg_eps_dynamic;
+  o.damping_alpha = opt.apply_damping ? m.damping_alpha : 0.0;
+  o.damping_beta = opt.apply_damping ? m.damping_beta : 0.0;
+  o.relax_reference = opt.relax_reference ? 1 : 0;
+  const int nrec = int(opt.receivers.size());
+  if (nrec)
+    c.check(femgpu_set_receivers(c.h, nrec, opt.receivers.data(), opt.receiver_quantity,
+                                 opt.receiver_every),
+            "set_receivers");
   femgpu_dyn_stats st{};
   c.check(femgpu_dyn_run(c.h, &o, dk.displacements().data(), &st), "dyn_run");
   if (PRINT_DEBUG_INFO)
     std::cout << "GN22: " << st.steps << " steps" << (m.beta2 == 0 ? " (explicit)" : " (implicit)")
               << ", " << st.pcg_iterations << " PCG iterations, " << st.run_ms << " ms\n";
+  if (nrec) {
+    // receivers.txt as the reference's receiver file (femfunc.cpp:226-243 header,
+    // writeDisplForSEGY rows, VTKfile.cpp:72-87) and receivers.sgy (convertToSEGY)
+    int ns = 0;
+    c.check(femgpu_get_traces(c.h, nullptr, 0, &ns), "get_traces");
+    const int d = dk.get_dim();
+    std::vector<double> tr(size_t(ns) * nrec * d);
+    c.check(femgpu_get_traces(c.h, tr.data(), int64_t(tr.size()), &ns), "get_traces");
+    const int ax = std::min(std::max(opt.receiver_axis, 0), d - 1);
+    const double dts = m.dt * opt.receiver_every;
+    const std::string dir = dk.getDataPaths().res_dir;
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    std::ofstream out(dir + "/receivers.txt");
+    if (!out) throw std::runtime_error("cannot write " + dir + "/receivers.txt");
+    out << std::setprecision(9) << nrec << "\t";
+    for (int r = 0; r < nrec; ++r) out << opt.receivers[r] << "\t";
+    out << "\n";
+    std::vector<float> seg(size_t(nrec) * ns);
+    for (int k = 0; k < ns; ++k) {
+      out << float((k + 1) * dts) << " ";
+      for (int r = 0; r < nrec; ++r) {
+        const double v = tr[(size_t(k) * nrec + r) * d + ax];
+        out << float(v) << " ";
+        seg[size_t(r) * ns + k] = float(v);
+      }
+      out << "\n";
+    }
+    c.check(femgpu_write_segy((dir + "/receivers.sgy").c_str(), seg.data(), nrec, ns, float(dts), 0),
+            "write_segy");
+  }
 }
 
 void WriteResults(dataKeeper &dk) {
+  std::error_code ec;
+  std::filesystem::create_directories(
+      std::filesystem::path(dk.getDataPaths().output_vtk).parent_path(), ec);
   if (dk.get_dim() == 2)
     MakeVTKfile2D(dk.getDataPaths().output_vtk, dk);
   else

@@ -11,6 +11,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "datakeeper.h"
 #include "femgpu.h"
@@ -32,6 +33,20 @@ struct SolverOptions {
   double pcg_eps_dynamic = 1e-4;  // femfunc.cpp:153
   bool compute_stress = true;     // per-element sigma for results.vtk
   int device = 0;
+  // endtime < 0: true = the reference's relaxation stub (one step), false =
+  // step until max |v| < |endtime| (femfunc.cpp:213-221,349-356)
+  bool relax_reference = true;
+  // Rayleigh damping from the run config (damping_alpha, damping_beta): the
+  // reference reads them but never applies them (isDamping = false,
+  // femfunc.cpp:151); false keeps that behaviour.
+  bool apply_damping = false;
+  // Receivers (femfunc.cpp:226-243,357-366, commented out there): node ids,
+  // recorded quantity, component written to receivers.txt / receivers.sgy,
+  // sampling period in steps.
+  std::vector<int32_t> receivers;
+  int receiver_quantity = FEMGPU_RECV_VELOCITY;
+  int receiver_axis = 1;
+  int receiver_every = 1;
 };
 
 void gpuCalculateFEM_EbE_vec(DEVICE_NAME device, dataKeeper &dk, bool PRINT_DEBUG_INFO,

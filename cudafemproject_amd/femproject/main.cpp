@@ -10,9 +10,15 @@
 //          --loads reference|physical  --mass reference|physical
 //          --implicit reference|physical  --no-stress  --device N
 //          --dry-run (read the problem, print its summary, no GPU)
+//          --relax (endtime < 0: step until max |v| < |endtime|)
+//          --damping (apply the config's damping_alpha / damping_beta)
+//          --receivers n1,n2,... --receiver-axis 0|1|2
+//          --receiver-quantity displacement|velocity|acceleration --segy-every K
+//          (dynamics: receivers.txt and receivers.sgy next to results.vtk)
 //
 // It reads the prepared mesh, runs the static or dynamic solve on the GPU
 // and writes results.vtk, as main.cpp:21-33 does.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
@@ -60,6 +66,29 @@ int parse_opts(int argc, char **argv, int first, SolverOptions &o, bool &tol_set
       g_dry_run = true;
     } else if (a == "--device") {
       o.device = std::atoi(val().c_str());
+    } else if (a == "--relax") {
+      o.relax_reference = false;
+    } else if (a == "--damping") {
+      o.apply_damping = true;
+    } else if (a == "--receivers") {
+      o.receivers.clear();
+      std::string v = val();
+      size_t p = 0;
+      while (p < v.size()) {
+        size_t q = v.find(',', p);
+        if (q == std::string::npos) q = v.size();
+        o.receivers.push_back(std::atoi(v.substr(p, q - p).c_str()));
+        p = q + 1;
+      }
+    } else if (a == "--receiver-axis") {
+      o.receiver_axis = std::atoi(val().c_str());
+    } else if (a == "--receiver-quantity") {
+      const std::string v = val();
+      o.receiver_quantity = v == "displacement" ? FEMGPU_RECV_DISPLACEMENT
+                            : v == "acceleration" ? FEMGPU_RECV_ACCELERATION
+                                                  : FEMGPU_RECV_VELOCITY;
+    } else if (a == "--segy-every") {
+      o.receiver_every = std::max(1, std::atoi(val().c_str()));
     } else {
       throw std::runtime_error("unknown option " + a);
     }

@@ -254,7 +254,7 @@ enum {
   FEMGPU_IMPLICIT_PHYSICAL = 1   /* solve (M + beta2 dt^2/2 K) a = F - K d     */
 };
 typedef struct {
-  double rho, beta1, beta2, dt, endtime;  /* endtime < 0: one step (relax stub) */
+  double rho, beta1, beta2, dt, endtime;  /* endtime < 0: relaxation, see below  */
   int wavelet_type;
   double wavelet_freq, wavelet_timeshift, wavelet_ampl;
   int precision;        /* FEMGPU_FP64 or FEMGPU_FP32 state                   */
@@ -262,6 +262,19 @@ typedef struct {
   int implicit_mode;    /* FEMGPU_IMPLICIT_REFERENCE / FEMGPU_IMPLICIT_PHYSICAL */
   double pcg_eps;       /* implicit inner PCG eps (1e-4 in femfunc.cpp:153)    */
   int steps_per_graph;  /* explicit: steps captured per HIP graph (0: 64)      */
+  /* Rayleigh damping C = alpha M + beta K (the reference's damping_alpha /
+   * damping_beta, femfunc.cpp:137-150,190-200,266-274, disabled there):
+   * explicit: beta must be 0, (1 + alpha beta1 dt) M a = F - K d~ - alpha M v~;
+   * implicit (FEMGPU_IMPLICIT_PHYSICAL only):
+   * ((1 + alpha beta1 dt) M + (beta2 dt^2/2 + beta beta1 dt) K) a
+   *     = F - K (d~ + beta v~) - alpha M v~                                  */
+  double damping_alpha, damping_beta;
+  /* endtime < 0 (relaxation, femfunc.cpp:213-221,349-356): with
+   * relax_reference != 0 one step, as the reference (its C-norm is a stub
+   * returning 0); else steps until max |v| < |endtime| (checked every
+   * steps_per_graph explicit steps / every implicit step), at most max_steps. */
+  int relax_reference;
+  int max_steps;
 } femgpu_dyn_opts;
 
 typedef struct {
@@ -274,6 +287,25 @@ typedef struct {
 int femgpu_default_dyn_opts(femgpu_dyn_opts *opts);
 int femgpu_dyn_run(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, double *u_out,
                    femgpu_dyn_stats *stats);
+
+/* Receivers (the reference's receiver traces for SEG-Y, writeDisplForSEGY,
+ * VTKfile.cpp:72-87, called every segy_step steps in femfunc.cpp:226-243,
+ * 357-366 where it is commented out): record the displacement, velocity or
+ * acceleration of the given nodes (user ids) at every `every`-th time step
+ * of the following femgpu_dyn_run calls; n = 0 clears.  Samples are taken
+ * after the step's correction, at times k * every * dt, k = 1, 2, ... */
+enum { FEMGPU_RECV_DISPLACEMENT = 0, FEMGPU_RECV_VELOCITY = 1, FEMGPU_RECV_ACCELERATION = 2 };
+int femgpu_set_receivers(femgpu_ctx *ctx, int n, const int32_t *nodes, int quantity, int every);
+/* Traces of the last femgpu_dyn_run: out[(k * n + r) * dim + c] for sample
+ * k < *n_samples, receiver r, component c; capacity in doubles.  out may be
+ * NULL to query *n_samples. */
+int femgpu_get_traces(femgpu_ctx *ctx, double *out, int64_t capacity, int *n_samples);
+/* SEG-Y file of n_traces traces of n_samples samples (traces[t * n_samples +
+ * s]) in the layout of the reference's convertToSEGY (VTKfile.cpp:89-138,
+ * format 5: big-endian IEEE floats, id in every trace header); host only,
+ * needs no device. */
+int femgpu_write_segy(const char *path, const float *traces, int n_traces, int n_samples,
+                      float sample_interval, int trace_id);
 
 #ifdef __cplusplus
 }

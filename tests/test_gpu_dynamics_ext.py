@@ -1,0 +1,203 @@
+"""GPU: receiver traces, SEG-Y output, Rayleigh damping and relaxation of the
+GN22 steppers (the parts of gpuCalculateFEM_DYN that are switched off in the
+reference, femfunc.cpp:137-151,213-243,349-366).  Each is checked against an
+independent consequence: traces against separate runs and the central
+difference identities of GN22 with beta1 = 1/2; damping against the modal
+decay law of mass-proportional damping and against the explicit stepper;
+relaxation against the static solve."""
+import json
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import cudafemproject_amd as F
+from conftest import GOLDEN, ROOT
+from cudafemproject_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def column(nz=16):
+    return P.column_problem(2, 2, nz, 0.1, 0.1, 0.1 * nz / 2, convention=F.LOADS_PHYSICAL,
+                            pressure=1.0)
+
+
+KW = dict(rho=1000.0, beta1=0.5, beta2=0.0, dt=2e-5, wavelet_type=F.WAVELET_RICKER, freq=500.0,
+          ampl=1e3, mass_convention=F.MASS_PHYSICAL)
+
+
+def test_traces_equal_separate_runs(ctx):
+    pb = column()
+    ctx.set_problem(pb)
+    recv = np.array([0, 5, pb.n_nodes // 2, pb.n_nodes - 1], np.int32)
+    ctx.set_receivers(recv, F.RECV_DISPLACEMENT, 1)
+    n = 80                                       # graph blocks of 64 and an eager tail
+    ctx.dyn_run(endtime=(n + 0.5) * KW["dt"], **KW)
+    tr = ctx.traces()
+    assert tr.shape == (n, len(recv), 3)
+    ctx.set_receivers([], F.RECV_DISPLACEMENT, 1)
+    for k in (1, 10, 64, n):
+        u, st = ctx.dyn_run(endtime=(k + 0.5) * KW["dt"], **KW)
+        assert st["steps"] == k
+        np.testing.assert_allclose(tr[k - 1], u[recv], rtol=1e-13, atol=1e-30)
+
+
+def test_velocity_acceleration_central_differences(ctx):
+    """GN22 with beta1 = 1/2, beta2 = 0: v_n = (d_{n+1} - d_{n-1}) / (2 dt) and
+    a_n = (d_{n+1} - 2 d_n + d_{n-1}) / dt^2 exactly (to rounding)."""
+    pb = column()
+    ctx.set_problem(pb)
+    recv = np.arange(0, pb.n_nodes, 7, dtype=np.int32)
+    out = {}
+    for q in (F.RECV_DISPLACEMENT, F.RECV_VELOCITY, F.RECV_ACCELERATION):
+        ctx.set_receivers(recv, q, 1)
+        ctx.dyn_run(endtime=100.5 * KW["dt"], **KW)
+        out[q] = ctx.traces()
+    d, v, a = out[F.RECV_DISPLACEMENT], out[F.RECV_VELOCITY], out[F.RECV_ACCELERATION]
+    dt = KW["dt"]
+    vc = (d[2:] - d[:-2]) / (2 * dt)
+    ac = (d[2:] - 2 * d[1:-1] + d[:-2]) / dt ** 2
+    assert np.abs(vc - v[1:-1]).max() <= 1e-8 * np.abs(v).max()
+    assert np.abs(ac - a[1:-1]).max() <= 1e-6 * np.abs(a).max()
+
+
+def test_sampling_period(ctx):
+    pb = column()
+    ctx.set_problem(pb)
+    recv = np.array([3, 17], np.int32)
+    ctx.set_receivers(recv, F.RECV_VELOCITY, 1)
+    ctx.dyn_run(endtime=90.5 * KW["dt"], **KW)
+    t1 = ctx.traces()
+    ctx.set_receivers(recv, F.RECV_VELOCITY, 7)
+    ctx.dyn_run(endtime=90.5 * KW["dt"], **KW)
+    t7 = ctx.traces()
+    assert t7.shape[0] == 90 // 7
+    np.testing.assert_array_equal(t7, t1[6::7][:t7.shape[0]])
+
+
+def lumped_mass(pb, rho):
+    X = pb.nodes[pb.elems]
+    V = np.abs(np.einsum("ei,ei->e", X[:, 1] - X[:, 0],
+                         np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))) / 6
+    m = np.zeros(pb.n_nodes)
+    np.add.at(m, pb.elems.ravel(), np.repeat(rho * V / 4, 4))
+    return m
+
+
+def energy(ctx, pb, d, v, rho):
+    m = lumped_mass(pb, rho)
+    return 0.5 * np.sum(m[:, None] * v * v) + 0.5 * np.sum(d * ctx.matvec(d, F.FP64))
+
+
+def run_energy(ctx, pb, T, **kw):
+    everything = np.arange(pb.n_nodes, dtype=np.int32)
+    steps = int(T / kw["dt"])
+    ctx.set_receivers(everything, F.RECV_VELOCITY, steps)
+    d, st = ctx.dyn_run(endtime=(steps + 0.5) * kw["dt"], **kw)
+    v = ctx.traces()[-1]
+    ctx.set_receivers([], F.RECV_VELOCITY, 1)
+    return energy(ctx, pb, d, v, kw["rho"]), d
+
+
+def test_zero_damping_is_undamped(ctx):
+    pb = column()
+    ctx.set_problem(pb)
+    a, _ = ctx.dyn_run(endtime=50.5 * KW["dt"], **KW)
+    b, _ = ctx.dyn_run(endtime=50.5 * KW["dt"], damping_alpha=0.0, damping_beta=0.0, **KW)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_mass_damping_decay_law(ctx):
+    """M a + alpha M v + K d = 0 after the pulse: every mode's energy decays as
+    exp(-alpha t) (up to terms of relative size alpha / (2 omega))."""
+    pb = column()
+    ctx.set_problem(pb)
+    T, alpha, t_pulse = 0.03, 40.0, 1.0 / KW["freq"]     # the Ricker peak
+    e0, _ = run_energy(ctx, pb, T, **KW)
+    ea, _ = run_energy(ctx, pb, T, damping_alpha=alpha, **KW)
+    ratio = ea / e0
+    expect = np.exp(-alpha * (T - t_pulse))
+    print(f"energy ratio {ratio:.4f} expected {expect:.4f}", flush=True)
+    assert abs(ratio / expect - 1) < 0.03
+
+
+def test_implicit_mass_damping_matches_explicit(ctx):
+    """Same damped problem through the implicit Newmark operator
+    ((1 + alpha beta1 dt) M + beta2 dt^2/2 K) and the explicit stepper: both
+    converge to the same motion as dt -> 0."""
+    pb = column(8)
+    ctx.set_problem(pb)
+    kw = dict(KW, dt=2e-6, freq=400.0)
+    T = 4e-3
+    ue, _ = ctx.dyn_run(endtime=T, damping_alpha=200.0, **kw)
+    ui, st = ctx.dyn_run(endtime=T, damping_alpha=200.0, implicit_mode=F.IMPLICIT_PHYSICAL,
+                         pcg_eps=1e-20, **dict(kw, beta2=0.5))
+    print(f"implicit vs explicit (damped) rel diff {rel(ui, ue):.2e}", flush=True)
+    assert rel(ui, ue) < 2e-4
+    u0, _ = ctx.dyn_run(endtime=T, implicit_mode=F.IMPLICIT_PHYSICAL, pcg_eps=1e-20,
+                        **dict(kw, beta2=0.5))
+    assert rel(u0, ue) > 5 * rel(ui, ue)        # damping visibly changes the motion
+
+
+def test_implicit_stiffness_damping_dissipates(ctx):
+    pb = column(8)
+    ctx.set_problem(pb)
+    kw = dict(KW, beta2=0.5, dt=1e-5, implicit_mode=F.IMPLICIT_PHYSICAL, pcg_eps=1e-16)
+    T = 5e-3
+    e = [run_energy(ctx, pb, T, damping_beta=b, **kw)[0] for b in (0.0, 1e-6, 1e-5)]
+    print("energies", e, flush=True)
+    assert e[0] > e[1] > e[2] > 0
+
+
+def test_dynamic_relaxation_reaches_static_solution(ctx):
+    """Constant load, mass damping near critical, relaxation until max|v| < eps:
+    the displacement is the static solution."""
+    pb = P.cube_problem(4, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)
+    ctx.set_problem(pb)
+    us, _ = ctx.solve(F.FP64, tol=1e-12, tol_kind=F.TOL_REL_RES)
+    u, st = ctx.dyn_run(rho=1000.0, beta1=0.5, beta2=0.0, dt=2e-5, endtime=-1e-9,
+                        mass_convention=F.MASS_PHYSICAL, damping_alpha=500.0,
+                        relax_reference=False, max_steps=400000, steps_per_graph=256)
+    print(f"relaxation: {st['steps']} steps, rel diff to static {rel(u, us):.2e}", flush=True)
+    assert st["status"] == 0 and st["steps"] < 400000
+    assert rel(u, us) < 1e-5
+
+
+def test_relaxation_reference_stub_is_one_step(ctx):
+    pb = column()
+    ctx.set_problem(pb)
+    _, st = ctx.dyn_run(endtime=-1.0, **KW)
+    assert st["steps"] == 1
+
+
+def test_fem_main_writes_receivers_and_segy(tmp_path):
+    from test_femproject_cpu import workspace
+    cfg = workspace(tmp_path, "test_rect_pcg", task_type="dynamic", rho=2400, dt=1e-5,
+                    endtime=2.05e-3, beta1=0.5, beta2=0.0,
+                    wavelet={"WaveletType": "ricker", "freq": 1000, "ampl": 1e3})
+    r = subprocess.run([os.path.join(ROOT, "cudafemproject_amd", "fem_main"), cfg, "test_rect_pcg",
+                        "--receivers", "1,3,5", "--receiver-axis", "0", "--segy-every", "4"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = tmp_path / "fem_results" / "2D" / "test_rect_pcg"
+    rows = (res / "receivers.txt").read_text().splitlines()
+    assert rows[0].split() == ["3", "1", "3", "5"]
+    vals = np.array([[float(x) for x in ln.split()] for ln in rows[1:]])
+    assert vals.shape == (205 // 4, 4)
+    np.testing.assert_allclose(vals[:, 0], 4e-5 * np.arange(1, 52), rtol=1e-6)
+    b = (res / "receivers.sgy").read_bytes()
+    ntr, ns = struct.unpack(">h", b[3212:3214])[0], struct.unpack(">h", b[3220:3222])[0]
+    assert (ntr, ns) == (3, 51) and struct.unpack(">h", b[3216:3218])[0] == 40
+    for t in range(3):
+        o = 3600 + t * (240 + 4 * ns) + 240
+        tr = np.frombuffer(b[o:o + 4 * ns], ">f4")
+        np.testing.assert_allclose(tr, vals[:, 1 + t], rtol=1e-6, atol=1e-30)
