@@ -297,6 +297,28 @@ def bench_batch(args, F):
     ctx.close()
 
 
+def traffic_probe(args, F):
+    """The command profiled for the HBM traffic figures: 1 + 5 FP32 cube
+    matvecs, then 3 device copies of a 256 MiB buffer as the counters'
+    calibration (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    from cudafemproject_amd.problem import cube_problem
+    pb = cube_problem(args.n, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)
+    ctx = F.Context(0)
+    ctx.set_problem(pb)
+    ctx.matvec_repeat(F.FP32, 1)
+    ctx.sync()
+    ctx.matvec_repeat(F.FP32, 5)
+    ctx.sync()
+    x = torch.ones(64 * 1024 * 1024, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("traffic probe done")
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["cube", "column", "cantilever", "batch"],
@@ -305,6 +327,8 @@ def main():
                          "dynamics; cantilever: compliance + adjoint gradient; batch: 512 "
                          "plates x 4 load cases in one launch")
     ap.add_argument("--dyn-steps", type=int, default=20000)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only the FP32 matvec + copy sequence profiled for HBM traffic")
     ap.add_argument("--fd-elems", type=int, default=64)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -321,6 +345,8 @@ def main():
 
     import cudafemproject_amd as F
     from cudafemproject_amd.problem import cube_problem
+    if args.traffic_probe:
+        return traffic_probe(args, F)
     if args.workload == "column":
         return bench_column(args, F)
     if args.workload == "cantilever":
