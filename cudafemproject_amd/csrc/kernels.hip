@@ -361,6 +361,27 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   }
 }
 
+// acc += the slots [j0, j1) of an interface node, in slot order, two loads
+// in flight per step.
+template <typename T, int DIM>
+__device__ __forceinline__ void add_slots(const T *partial, int j0, int j1, T *acc) {
+  int j = j0;
+  for (; j + 2 <= j1; j += 2) {
+    T u[DIM], w[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      u[c] = partial[int64_t(j) * DIM + c];
+      w[c] = partial[int64_t(j + 1) * DIM + c];
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] = (acc[c] + u[c]) + w[c];
+  }
+  if (j < j1) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc[c] += partial[int64_t(j) * DIM + c];
+  }
+}
+
 // Interface gather: y[iface] = sum of slots (fixed order).
 template <typename T, int DIM>
 __global__ __launch_bounds__(NT) void k_iface_gather(int64_t n_owned, int64_t n_iface,
@@ -371,9 +392,7 @@ __global__ __launch_bounds__(NT) void k_iface_gather(int64_t n_owned, int64_t n_
   T acc[DIM];
 #pragma unroll
   for (int c = 0; c < DIM; ++c) acc[c] = T(0);
-  for (int j = slot_start[s]; j < slot_start[s + 1]; ++j)
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] += partial[int64_t(j) * DIM + c];
+  add_slots<T, DIM>(partial, slot_start[s], slot_start[s + 1], acc);
 #pragma unroll
   for (int c = 0; c < DIM; ++c) y[(n_owned + s) * DIM + c] = acc[c];
 }
@@ -436,9 +455,7 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
       const int64_t s = i - a.n_owned;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) qv[c] = T(0);
-      for (int j = a.slot_start[s]; j < a.slot_start[s + 1]; ++j)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) qv[c] += a.partial[int64_t(j) * DIM + c];
+      add_slots<T, DIM>(a.partial, a.slot_start[s], a.slot_start[s + 1], qv);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         pv[c] = a.z[i * DIM + c] + beta * a.p[i * DIM + c];
@@ -814,9 +831,7 @@ __global__ __launch_bounds__(NT) void k_dyn_correct(int64_t n_owned, int64_t n_a
       const int64_t s = i - n_owned;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) kd[c] = T(0);
-      for (int j = slot_start[s]; j < slot_start[s + 1]; ++j)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) kd[c] += partial[int64_t(j) * DIM + c];
+      add_slots<T, DIM>(partial, slot_start[s], slot_start[s + 1], kd);
     }
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
