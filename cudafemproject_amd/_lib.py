@@ -24,6 +24,7 @@ STATUS_NAMES = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_NONFINITE", -4: "E_MAX
 
 FP64, FP32, MIXED = 0, 1, 2
 TOL_REF_GAMMA, TOL_REL_RES = 0, 1
+CG_STANDARD, CG_SINGLE_REDUCTION = 0, 1
 LOADS_REFERENCE, LOADS_PHYSICAL = 0, 1
 MASS_REFERENCE, MASS_PHYSICAL = 0, 1
 WAVELET_NONE, WAVELET_RICKER, WAVELET_BERLAGE = 0, 1, 2
@@ -33,7 +34,8 @@ RECV_DISPLACEMENT, RECV_VELOCITY, RECV_ACCELERATION = 0, 1, 2
 
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("tol_kind", C.c_int), ("tol", C.c_double),
-                ("max_iter", C.c_int), ("inner_tol", C.c_double), ("check_every", C.c_int)]
+                ("max_iter", C.c_int), ("inner_tol", C.c_double), ("check_every", C.c_int),
+                ("recurrence", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -279,11 +281,12 @@ class Context:
         self._check(self.lib.femgpu_sync(self.h), "sync")
 
     # --- solves ----------------------------------------------------------------
-    def _opts(self, precision, tol, tol_kind, max_iter, inner_tol, check_every) -> SolveOpts:
+    def _opts(self, precision, tol, tol_kind, max_iter, inner_tol, check_every,
+              recurrence=CG_STANDARD) -> SolveOpts:
         o = SolveOpts()
         self.lib.femgpu_default_solve_opts(C.byref(o))
         o.precision, o.tol, o.tol_kind, o.max_iter = precision, tol, tol_kind, max_iter
-        o.inner_tol, o.check_every = inner_tol, check_every
+        o.inner_tol, o.check_every, o.recurrence = inner_tol, check_every, recurrence
         return o
 
     @staticmethod
@@ -294,8 +297,8 @@ class Context:
 
     def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
               max_iter: int = 0, inner_tol: float = 0.1, check_every: int = 32,
-              raise_on_error: bool = True):
-        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every)
+              raise_on_error: bool = True, recurrence: int = CG_STANDARD):
+        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every, recurrence)
         u = np.zeros((self.n_nodes, self.dim))
         s = Stats()
         st = self.lib.femgpu_solve_static(self.h, C.byref(o), _dp(u), C.byref(s))
@@ -305,10 +308,10 @@ class Context:
 
     def solve_rhs(self, f: np.ndarray, precision: int = FP64, tol: float = 1e-8,
                   tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
-                  check_every: int = 32):
+                  check_every: int = 32, recurrence: int = CG_STANDARD):
         """Solve K u = f for an explicit right-hand side (n_nodes x dim)."""
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n_nodes, self.dim)
-        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every)
+        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every, recurrence)
         u = np.zeros((self.n_nodes, self.dim))
         s = Stats()
         self._check(self.lib.femgpu_solve_rhs(self.h, C.byref(o), _dp(f), _dp(u), C.byref(s)),
@@ -325,9 +328,9 @@ class Context:
 
     def compliance_grad(self, precision: int = FP64, tol: float = 1e-8,
                         tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
-                        check_every: int = 32):
+                        check_every: int = 32, recurrence: int = CG_STANDARD):
         """C = f'u and dC/ds_e from one solve; returns (C, g, u, stats)."""
-        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every)
+        o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every, recurrence)
         u = np.zeros((self.n_nodes, self.dim))
         g = np.zeros(self.n_elems)
         c = C.c_double(0)

@@ -1,0 +1,484 @@
+// cg1.hip -- single-reduction Jacobi-PCG: one launch per iteration.
+//
+// The reference's PCG loop (femfunc.cpp:59-111) needs two global reductions
+// per iteration (p'Ap for alpha, r'z for beta), i.e. two grid-wide
+// dependencies and two launches here (k_ebe_patch MODE 1 + k_pcg_update).
+// This is the same Jacobi-PCG in the Chronopoulos-Gear form (A. T.
+// Chronopoulos, C. W. Gear, "s-step iterative methods for symmetric linear
+// systems", J. Comput. Appl. Math. 25 (1989)), which carries s = A p and
+// w = A u (u = M^-1 r) as recurrences so that both inner products of an
+// iteration come from one reduction:
+//   p_j = u_j + b_j p_{j-1},  s_j = w_j + b_j s_{j-1},
+//   x_{j+1} = x_j + a_j p_j,  r_{j+1} = r_j - a_j s_j,  u_{j+1} = M^-1 r_{j+1},
+//   w_{j+1} = A u_{j+1},  g_{j+1} = r_{j+1}'u_{j+1},  d_{j+1} = w_{j+1}'u_{j+1},
+//   b_{j+1} = g_{j+1} / g_j,  a_{j+1} = g_{j+1} / (d_{j+1} - b_{j+1} g_{j+1} / a_j).
+// In exact arithmetic the iterates equal the reference's.
+//
+// One launch = one iteration, one workgroup per patch (patch_common.h):
+//  * preamble: every block reduces the previous launch's per-patch partials
+//    of g, d and r'r in the same fixed order, runs the stop test and forms
+//    a_j, b_j;
+//  * phase A, one patch node per thread: w_j of the node (owned: the vector;
+//    interface: the fixed-order sum of its slots), then s_j, r_{j+1},
+//    u_{j+1}.  Every patch that touches an interface node computes these
+//    (bitwise the same values: same inputs, same order); the node's first
+//    patch also forms p_j, x_{j+1} and writes the node's state;
+//  * phases B and C: w_{j+1} = A u_{j+1} (owned nodes in place, interface
+//    nodes into slots) and the patch's partial of d_{j+1} = sum_e u_e'K_e u_e
+//    (+ constrained diagonal terms); g and r'r partials over the nodes the
+//    patch writes.
+// The node state (x, r, p, s) and the slots ping-pong by the launch's parity,
+// so no block reads a value another block of the same launch writes.  The
+// parity follows the launches (every graph replay starts at parity 0), not
+// the iteration count: after a residual replacement the two differ.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "cg1.h"
+#include "element.h"
+#include "patch_common.h"
+
+namespace femgpu {
+
+namespace {
+
+using namespace elem;
+using namespace patch;
+
+template <typename T>
+struct Cg1Args {
+  const PatchInfo *patches;
+  const uint32_t *erec;
+  const int4 *sh_info;  // per patch interface node: id, slot, node's first slot, end slot
+  const uint16_t *node_end;
+  const uint8_t *fixmask;
+  const T *coords, *escale, *diagK, *invD;
+  T lam, mu;
+  int np, p8, ent, pn;
+  Cg1State<T> st;
+  CgScalars *sc;
+  int parity;
+};
+
+// INIT: w = A u, u = M^-1 r[0], into w / slot[0], and the partials of g, d,
+// r'r into part[0] (the state the first iteration launch reads).
+template <typename T, int DIM, int PT, bool INIT>
+__global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
+  constexpr unsigned FMASK = (1u << DIM) - 1u;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int b = blockIdx.x;
+  // XCD-aware: consecutive patches share interface nodes and an L2
+  const int pid = (b & 7) * a.p8 + (b >> 3);
+  if (pid >= a.np) return;
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.ent);
+  T *s_c = reinterpret_cast<T *>(lds + L.off_c);
+  T *s_x = reinterpret_cast<T *>(lds + L.off_x);      // u, constrained components zeroed
+  T *s_res = reinterpret_cast<T *>(lds + L.off_res);  // one component of the entries
+  double *s_red = reinterpret_cast<double *>(lds + L.off_red);
+  const int in = INIT ? 0 : a.parity;
+  const int out = INIT ? 0 : a.parity ^ 1;
+  const Cg1State<T> &S = a.st;
+
+  T alpha = T(0), beta = T(0);
+  if constexpr (!INIT) {
+    CgScalars *sc = a.sc;
+    if (sc->done) return;
+    const double *pin = S.part + size_t(in) * 3 * a.np;
+    double g = 0.0, d = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < a.np; i += PT) {
+      g += pin[i];
+      d += pin[a.np + i];
+      q += pin[2 * a.np + i];
+    }
+    block_sum3<PT>(g, d, q, s_red);
+    const int q1 = in ^ 1;
+    const int j = sc->it[q1] + 1;
+    int stop = 0;
+    if (j >= 1) {
+      const bool conv = sc->kind == 0 ? (g < sc->thresh) : (q <= sc->thresh);
+      if (conv) stop = 1;
+    } else if (q == 0.0) {
+      stop = 1;
+    }
+    if (!stop && !isfinite(g)) stop = 2;
+    if (!stop && j >= sc->max_iter) stop = 4;
+    if (!stop && q < sc->pause) stop = 5;
+    double bt = 0.0, al = 0.0;
+    if (!stop) {
+      // b_j and a_j; g[q1] = a[q1] = +inf at a (re)start gives b = 0, a = g/d
+      bt = g / sc->g[q1];
+      const double den = d - bt * g / sc->al[q1];
+      al = g / den;
+      if (!(den > 0.0) || !isfinite(al)) stop = isfinite(den) ? 3 : 2;
+    }
+    if (pid == 0 && threadIdx.x == 0) {
+      sc->g_final = g;
+      sc->rr_final = q;
+      sc->iter_final = j;
+      if (stop) {
+        sc->done = stop;
+        sc->par_final = in;
+      } else {
+        sc->g[in] = g;
+        sc->al[in] = al;
+        sc->b[in] = bt;
+        sc->it[in] = j;
+        sc->pq = d;
+      }
+    }
+    if (stop) return;
+    alpha = T(al);
+    beta = T(bt);
+  }
+
+  const PatchInfo P = a.patches[pid];
+  const int nloc = P.n_own + P.n_sh;
+  uint3 rec[EPT];
+  T esc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * PT;
+    rec[e] = i < P.n_elem ? reinterpret_cast<const uint3 *>(a.erec)[P.elem_start + i]
+                          : make_uint3(0u, 0u, 0u);
+    esc[e] = a.escale && i < P.n_elem ? a.escale[P.elem_start + i] : T(1);
+  }
+  clear_entries<T, PT>(s_res, a.ent);
+
+  // Phase A
+  const int k = threadIdx.x;
+  const bool has_node = k < nloc;
+  const bool own = k < P.n_own;
+  bool first = own;
+  int64_t gid = 0, slot = 0;
+  int b0 = 0, b1 = 0;
+  unsigned fm = 0;
+  T u[DIM];
+  // this node's new state (first patch only); FP32: stored after the
+  // assembly so that the phase barriers do not wait for these writes (FP64:
+  // at once, the registers are needed by the element math)
+  constexpr bool LATE = sizeof(T) == 4;
+  T np_[DIM], ns_[DIM], nx_[DIM], nr_[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) u[c] = np_[c] = ns_[c] = nx_[c] = nr_[c] = T(0);
+  double acc_g = 0.0, acc_d = 0.0, acc_q = 0.0;
+  if (has_node) {
+    int s0 = 0, s1 = 0;
+    if (own) {
+      gid = int64_t(P.own_start + k);
+    } else {
+      const int4 si = a.sh_info[P.sh_start + k - P.n_own];
+      gid = si.x;
+      slot = si.y;
+      s0 = si.z;
+      s1 = si.w;
+      first = si.y == si.z;
+    }
+    b1 = int(a.node_end[P.pnode_start + k]);
+    b0 = k ? int(a.node_end[P.pnode_start + k - 1]) : 0;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) s_c[k * DIM + c] = a.coords[gid * DIM + c];
+    fm = a.fixmask[gid] & FMASK;
+    const int64_t o = gid * DIM;
+    T rn[DIM];
+    if constexpr (INIT) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        rn[c] = S.r[0][o + c];
+        u[c] = a.invD[o + c] * rn[c];
+      }
+    } else {
+      T rv[DIM], sv[DIM], dv[DIM], wv[DIM], pv[DIM], xv[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        rv[c] = S.r[in][o + c];
+        sv[c] = S.s[in][o + c];
+        dv[c] = a.invD[o + c];
+        pv[c] = T(0);
+        xv[c] = T(0);
+      }
+      if (own) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) wv[c] = S.w[o + c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) wv[c] = T(0);
+        add_slots<T, DIM>(S.slot[in], s0, s1, wv);
+      }
+      if (first) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          pv[c] = S.p[in][o + c];
+          xv[c] = S.x[in][o + c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const T sn = wv[c] + beta * sv[c];
+        rn[c] = rv[c] - alpha * sn;
+        u[c] = dv[c] * rn[c];
+        const T pn = dv[c] * rv[c] + beta * pv[c];
+        np_[c] = pn;
+        ns_[c] = sn;
+        nx_[c] = xv[c] + alpha * pn;
+        nr_[c] = rn[c];
+        if (!LATE && first) {
+          S.p[out][o + c] = np_[c];
+          S.s[out][o + c] = ns_[c];
+          S.x[out][o + c] = nx_[c];
+          S.r[out][o + c] = nr_[c];
+        }
+      }
+    }
+    if (first) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        acc_g += double(rn[c]) * double(u[c]);
+        acc_q += double(rn[c]) * double(rn[c]);
+        if ((fm >> c) & 1u) acc_d += double(a.diagK[o + c]) * double(u[c]) * double(u[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) s_x[k * DIM + c] = ((fm >> c) & 1u) ? T(0) : u[c];
+  }
+  __syncthreads();
+
+  // Phases B and C: w_{j+1} = A u_{j+1}
+  T acc[DIM];
+  assemble<T, DIM, PT, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0, b1, acc, acc_d,
+                              1.0);
+  if (has_node) {
+    // constrained rows: the assembled diagonal only (applyConstraints,
+    // cpu_fem_utils.cpp:461-476), written by the node's first patch
+    if (fm) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        if ((fm >> c) & 1u) acc[c] = first ? a.diagK[gid * DIM + c] * u[c] : T(0);
+    }
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) S.w[gid * DIM + c] = acc[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) S.slot[out][slot * DIM + c] = acc[c];
+    }
+    if (!INIT && LATE && first) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        S.p[out][gid * DIM + c] = np_[c];
+        S.s[out][gid * DIM + c] = ns_[c];
+        S.x[out][gid * DIM + c] = nx_[c];
+        S.r[out][gid * DIM + c] = nr_[c];
+      }
+    }
+  }
+  block_sum3<PT>(acc_g, acc_d, acc_q, s_red);
+  if (threadIdx.x == 0) {
+    double *po = S.part + size_t(out) * 3 * a.np;
+    po[pid] = acc_g;
+    po[a.np + pid] = acc_d;
+    po[2 * a.np + pid] = acc_q;
+  }
+}
+
+constexpr int DOT_BLOCKS = 512, DOT_THREADS = 256;
+
+// Partials of s'u and p's with u = M^-1 r (u formed as in k_cg1), for the
+// exact step length after a residual replacement.
+template <typename T>
+__global__ __launch_bounds__(DOT_THREADS) void k_cg1_dots(int64_t n, const T *r, const T *p,
+                                                          const T *s, const T *invD,
+                                                          double *out) {
+  __shared__ double s_red[2 * DOT_THREADS / 64];
+  double su = 0.0, ps = 0.0;
+  const int64_t stride = int64_t(gridDim.x) * DOT_THREADS;
+  for (int64_t i = int64_t(blockIdx.x) * DOT_THREADS + threadIdx.x; i < n; i += stride) {
+    const T u = invD[i] * r[i];
+    su += double(s[i]) * double(u);
+    ps += double(p[i]) * double(s[i]);
+  }
+  block_sum2<DOT_THREADS>(su, ps, s_red);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = su;
+    out[DOT_BLOCKS + blockIdx.x] = ps;
+  }
+}
+
+// Scalars after an INIT launch.  mode 0: fresh start (g0, thresholds,
+// counters); 1: residual replacement keeping the search direction p, state
+// moved from parity q; 2: replacement with a restart (b = 0 next).
+// Mode 1 sets the stored previous step length so that the next launch's
+// denominator d - b g / a_prev equals p'A p of the new direction
+// p = u + b p_prev exactly: p'A p = d + 2 b s_prev'u + b^2 p_prev's_prev
+// (dots: the k_cg1_dots partials), instead of relying on the recurrence
+// identities that the replaced residual no longer satisfies.
+__global__ void k_cg1_fin(CgScalars *sc, const double *part, int np, int mode, int q,
+                          const double *dots, double pause_drop) {
+  __shared__ double s_red[3 * 256 / 64];
+  double g = 0.0, d = 0.0, rr = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    g += part[i];
+    d += part[np + i];
+    rr += part[2 * np + i];
+  }
+  block_sum3<256>(g, d, rr, s_red);
+  double su = 0.0, ps = 0.0;
+  if (mode == 1) {
+    for (int i = threadIdx.x; i < DOT_BLOCKS; i += 256) {
+      su += dots[i];
+      ps += dots[DOT_BLOCKS + i];
+    }
+    block_sum2<256>(su, ps, s_red);
+  }
+  if (threadIdx.x != 0) return;
+  const double inf = INFINITY;
+  sc->g_final = g;
+  sc->rr_final = rr;
+  sc->pause = pause_drop > 0.0 ? pause_drop * pause_drop * rr : 0.0;
+  if (mode == 0) {
+    sc->gamma0 = g;
+    sc->rr0 = rr;
+    sc->g[0] = sc->g[1] = inf;
+    sc->al[0] = sc->al[1] = inf;
+    sc->b[0] = sc->b[1] = 0.0;
+    sc->it[0] = sc->it[1] = -1;
+    sc->iter_final = 0;
+    sc->par_final = 0;
+    sc->thresh = sc->kind == 0 ? sc->eps * g : sc->eps * sc->eps * rr;
+    sc->done = isfinite(g) && isfinite(d) ? 0 : 2;
+    return;
+  }
+  if (q) {
+    sc->g[1] = sc->g[0];
+    sc->al[1] = sc->al[0];
+    sc->b[1] = sc->b[0];
+    sc->it[1] = sc->it[0];
+  }
+  if (mode == 2) {
+    sc->g[1] = sc->al[1] = inf;
+  } else {
+    const double bt = g / sc->g[1];
+    const double pap = d + 2.0 * bt * su + bt * bt * ps;
+    sc->al[1] = (bt == 0.0 || pap == d) ? inf : bt * g / (d - pap);
+  }
+  if (sc->done == 1 || sc->done == 5) sc->done = 0;
+}
+
+template <typename T>
+Cg1Args<T> cg1_args(const DevMesh &dm, const Cg1State<T> &st, const T *invD, CgScalars *sc,
+                    int parity) {
+  Cg1Args<T> a{};
+  a.patches = dm.patches;
+  a.erec = dm.erec;
+  a.sh_info = dm.sh_info;
+  a.node_end = dm.node_end;
+  a.fixmask = dm.fixmask;
+  if constexpr (sizeof(T) == 8) {
+    a.coords = reinterpret_cast<const T *>(dm.coords64);
+    a.escale = reinterpret_cast<const T *>(dm.escale64);
+    a.diagK = reinterpret_cast<const T *>(dm.diag64);
+  } else {
+    a.coords = reinterpret_cast<const T *>(dm.coords32);
+    a.escale = reinterpret_cast<const T *>(dm.escale32);
+    a.diagK = reinterpret_cast<const T *>(dm.diag32);
+  }
+  a.invD = invD;
+  a.lam = T(dm.lam);
+  a.mu = T(dm.mu);
+  a.np = dm.np;
+  a.p8 = (dm.np + 7) / 8;
+  a.ent = dm.ent;
+  a.pn = dm.pn;
+  a.st = st;
+  a.sc = sc;
+  a.parity = parity;
+  return a;
+}
+
+template <typename T, int DIM, int PT, bool INIT>
+void launch_pt(const DevMesh &dm, const Cg1Args<T> &a, hipStream_t s) {
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.ent);
+  hipLaunchKernelGGL((k_cg1<T, DIM, PT, INIT>), dim3(8 * a.p8), dim3(PT), L.total, s, a);
+}
+
+template <typename T, bool INIT>
+void launch(const DevMesh &dm, const Cg1Args<T> &a, hipStream_t s) {
+  if (dm.dim == 3) {
+    if (dm.patch_threads == 1024)
+      launch_pt<T, 3, 1024, INIT>(dm, a, s);
+    else if (dm.patch_threads == 512)
+      launch_pt<T, 3, 512, INIT>(dm, a, s);
+    else
+      launch_pt<T, 3, 256, INIT>(dm, a, s);
+  } else {
+    if (dm.patch_threads == 1024)
+      launch_pt<T, 2, 1024, INIT>(dm, a, s);
+    else if (dm.patch_threads == 512)
+      launch_pt<T, 2, 512, INIT>(dm, a, s);
+    else
+      launch_pt<T, 2, 256, INIT>(dm, a, s);
+  }
+}
+
+template <typename T, int DIM, int PT, bool INIT>
+hipError_t attr() {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cg1<T, DIM, PT, INIT>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+}  // namespace
+
+hipError_t prepare_cg1_kernels() {
+  hipError_t e = hipSuccess, r;
+#define FEMGPU_CG1_ATTR(T, D)                                                         \
+  if ((r = attr<T, D, 256, false>()) != hipSuccess) e = r;                            \
+  if ((r = attr<T, D, 512, false>()) != hipSuccess) e = r;                            \
+  if ((r = attr<T, D, 1024, false>()) != hipSuccess) e = r;                           \
+  if ((r = attr<T, D, 256, true>()) != hipSuccess) e = r;                             \
+  if ((r = attr<T, D, 512, true>()) != hipSuccess) e = r;                             \
+  if ((r = attr<T, D, 1024, true>()) != hipSuccess) e = r;
+  FEMGPU_CG1_ATTR(double, 3)
+  FEMGPU_CG1_ATTR(double, 2)
+  FEMGPU_CG1_ATTR(float, 3)
+  FEMGPU_CG1_ATTR(float, 2)
+#undef FEMGPU_CG1_ATTR
+  return e;
+}
+
+template <typename T>
+void launch_cg1_init(const DevMesh &dm, const Cg1State<T> &st, const T *invD, hipStream_t s) {
+  launch<T, true>(dm, cg1_args<T>(dm, st, invD, nullptr, 0), s);
+}
+
+template <typename T>
+void launch_cg1_iter(const DevMesh &dm, const Cg1State<T> &st, const T *invD, CgScalars *sc,
+                     int parity, hipStream_t s) {
+  launch<T, false>(dm, cg1_args<T>(dm, st, invD, sc, parity), s);
+}
+
+template <typename T>
+void launch_cg1_dots(int64_t n, const T *r, const T *p, const T *s_, const T *invD, double *dots,
+                     hipStream_t s) {
+  hipLaunchKernelGGL((k_cg1_dots<T>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, r, p, s_, invD,
+                     dots);
+}
+
+void launch_cg1_fin(CgScalars *sc, const double *part, int np, int mode, int q,
+                    const double *dots, double pause_drop, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg1_fin, dim3(1), dim3(256), 0, s, sc, part, np, mode, q, dots,
+                     pause_drop);
+}
+
+template void launch_cg1_dots<float>(int64_t, const float *, const float *, const float *,
+                                     const float *, double *, hipStream_t);
+
+template void launch_cg1_init<double>(const DevMesh &, const Cg1State<double> &, const double *,
+                                      hipStream_t);
+template void launch_cg1_init<float>(const DevMesh &, const Cg1State<float> &, const float *,
+                                     hipStream_t);
+template void launch_cg1_iter<double>(const DevMesh &, const Cg1State<double> &, const double *,
+                                      CgScalars *, int, hipStream_t);
+template void launch_cg1_iter<float>(const DevMesh &, const Cg1State<float> &, const float *,
+                                     CgScalars *, int, hipStream_t);
+
+}  // namespace femgpu

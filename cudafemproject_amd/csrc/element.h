@@ -60,6 +60,32 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
   b = rb;
 }
 
+// Three block-wide sums with one pair of barriers; sh holds 3 * BT / 64 doubles.
+template <int BT>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *sh) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+    sh[3 * w] = a;
+    sh[3 * w + 1] = b;
+    sh[3 * w + 2] = c;
+  }
+  __syncthreads();
+  double ra = 0, rb = 0, rc = 0;
+#pragma unroll
+  for (int i = 0; i < BT / 64; ++i) {
+    ra += sh[3 * i];
+    rb += sh[3 * i + 1];
+    rc += sh[3 * i + 2];
+  }
+  a = ra;
+  b = rb;
+  c = rc;
+}
+
 // ---------------------------------------------------------------------------
 // Element kernels (registers only).  X, U: node-major (npe x dim).
 // fix: bit (dim*a + c) set when that local DOF is constrained.

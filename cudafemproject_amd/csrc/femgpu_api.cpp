@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -17,6 +18,7 @@
 
 #include "femgpu.h"
 #include "batch.h"
+#include "cg1.h"
 #include "femgpu_internal.h"
 #include "kernels.h"
 
@@ -493,6 +495,88 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
   return FEMGPU_OK;
 }
 
+// Single-reduction PCG (cg1.hip): device state of one precision.
+template <typename T>
+int cg1_ws(femgpu_ctx *ctx, const char *tag, femgpu::Cg1State<T> &w) {
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  const size_t ns = size_t(ctx->plan.n_slots) * ctx->dim + 1;
+  const std::string t = std::string("cg1") + tag;
+  int st;
+  for (int k = 0; k < 2; ++k) {
+    const std::string i = std::to_string(k);
+    if ((st = dbuf(ctx, t + "x" + i, nd, &w.x[k])) || (st = dbuf(ctx, t + "r" + i, nd, &w.r[k])) ||
+        (st = dbuf(ctx, t + "p" + i, nd, &w.p[k])) || (st = dbuf(ctx, t + "s" + i, nd, &w.s[k])) ||
+        (st = dbuf(ctx, t + "slot" + i, ns, &w.slot[k])))
+      return st;
+  }
+  if ((st = dbuf(ctx, t + "w", nd, &w.w)) ||
+      (st = dbuf(ctx, "cg1part", size_t(6) * std::max(ctx->dm.np, 1), &w.part)))
+    return st;
+  return FEMGPU_OK;
+}
+
+// Graph of `block` single-reduction iterations (block even: every replay
+// starts at parity 0); captured once per buffer set and cached.
+template <typename T>
+int cg1_graph(femgpu_ctx *ctx, const femgpu::Cg1State<T> &w, const T *invD, int block,
+              int prec_tag, CgScalars *sc, hipGraphExec_t *out) {
+  if (ctx->graph_gen != ctx->buf_gen) {
+    drop_graphs(ctx);
+    ctx->graph_gen = ctx->buf_gen;
+  }
+  char keybuf[160];
+  std::snprintf(keybuf, sizeof(keybuf), "cg1/%d/%d/%p/%p/%p/%p", prec_tag, block,
+                (const void *)w.x[0], (const void *)invD, (const void *)sc, (const void *)w.part);
+  const std::string key(keybuf);
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end()) {
+    *out = it->second;
+    return FEMGPU_OK;
+  }
+  hipGraph_t g;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < block; ++k)
+    femgpu::launch_cg1_iter<T>(ctx->dm, w, invD, sc, k & 1, ctx->stream);
+  HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+  HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  ctx->graphs[key] = exec;
+  *out = exec;
+  return FEMGPU_OK;
+}
+
+// Iterations done and the parity of the state once the stop flag is set:
+// the launch that sets it does no iteration and its parity is the state's.
+struct Cg1Pos {
+  int iters, q;
+};
+Cg1Pos cg1_pos(const CgScalars &h) { return {h.iter_final, h.par_final}; }
+
+// Runs the single-reduction PCG from r[0] with x[0], p[0], s[0] cleared
+// here; on return x[q] holds the solution.
+template <typename T>
+int run_cg1(femgpu_ctx *ctx, femgpu::Cg1State<T> &w, const T *invD, int kind, double tol,
+            int max_iter, int block, int prec_tag, CgScalars *sc, Cg1Pos *pos, int *done_code) {
+  const int64_t nd = ctx->nn * ctx->dim;
+  femgpu::launch_fill<T>(w.x[0], nd, T(0), ctx->stream);
+  femgpu::launch_fill<T>(w.p[0], nd, T(0), ctx->stream);
+  femgpu::launch_fill<T>(w.s[0], nd, T(0), ctx->stream);
+  femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
+  femgpu::launch_cg1_init<T>(ctx->dm, w, invD, ctx->stream);
+  femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, nullptr, 0.0, ctx->stream);
+  HIPCHK(hipGetLastError());
+  block = std::max(2, block + (block & 1));
+  hipGraphExec_t exec;
+  int st;
+  if ((st = cg1_graph<T>(ctx, w, invD, block, prec_tag, sc, &exec)) ||
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0)))
+    return st;
+  *pos = cg1_pos(*ctx->h_sc);
+  *done_code = ctx->h_sc->done;
+  return FEMGPU_OK;
+}
+
 int done_status(int done) {
   switch (done) {
     case 1: return FEMGPU_OK;
@@ -558,6 +642,7 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
   HIPCHK(femgpu::prepare_kernels());
+  HIPCHK(femgpu::prepare_cg1_kernels());
   *out = c.release();
   return FEMGPU_OK;
 }
@@ -617,8 +702,19 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   const int threads = ctx->threads_req ? ctx->threads_req : 1024;
   int emax = ctx->emax > 0 ? ctx->emax : 3 * threads;
   if (emax > 3 * threads) return fail(ctx, FEMGPU_E_ARG, "elements per patch above 3 x patch threads");
-  std::string msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, threads, true, ctx->plan);
-  if (!msg.empty()) return fail(ctx, FEMGPU_E_BAD_MESH, msg);
+  // The FP64 patch kernels must fit a workgroup's LDS (160 KiB on gfx950):
+  // patches with many nodes per element (thin or irregular meshes) are
+  // re-planned with fewer elements.
+  constexpr int LDS_MAX = 160 * 1024;
+  std::string msg;
+  for (;;) {
+    msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, threads, true, ctx->plan);
+    if (!msg.empty()) return fail(ctx, FEMGPU_E_BAD_MESH, msg);
+    const int ent = (ctx->plan.ent_used + 3) & ~3;
+    if (femgpu::lds_bytes(dim, 8, ctx->plan.pn_used, ent) <= LDS_MAX) break;
+    if (emax <= 64) return fail(ctx, FEMGPU_E_BAD_MESH, "patch does not fit the LDS");
+    emax = std::max(64, emax * 7 / 8);
+  }
   // degenerate elements
   for (int64_t e = 0; e < n_elems; ++e) {
     const int32_t *en = &elems[e * npe];
@@ -693,6 +789,20 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   dm.erec = rec;
   dm.sh_gid = shg;
   dm.sh_slot = shs;
+  {
+    // per patch interface node: id, own slot, the node's slot range
+    std::vector<int32_t> info(P.sh_gid.size() * 4);
+    for (size_t i = 0; i < P.sh_gid.size(); ++i) {
+      const int64_t s = P.sh_gid[i] - P.n_owned;
+      info[4 * i + 0] = P.sh_gid[i];
+      info[4 * i + 1] = P.sh_slot[i];
+      info[4 * i + 2] = P.slot_start[s];
+      info[4 * i + 3] = P.slot_start[s + 1];
+    }
+    int32_t *pi;
+    if ((st = upload(ctx, "sh_info", info, &pi))) return st;
+    dm.sh_info = reinterpret_cast<const int4 *>(pi);
+  }
   dm.node_end = ne_;
   dm.slot_start = ss;
   dm.elems_int = ei;
@@ -934,6 +1044,147 @@ int femgpu_default_solve_opts(femgpu_solve_opts *o) {
   o->max_iter = 0;
   o->inner_tol = 0.1;
   o->check_every = 32;
+  o->recurrence = FEMGPU_CG_STANDARD;
+  return FEMGPU_OK;
+}
+
+// FEMGPU_TRACE=1 in the environment: the mixed solves print one line per
+// FP64 residual replacement to stderr.
+static bool trace_enabled() {
+  static const bool on = [] {
+    const char *v = std::getenv("FEMGPU_TRACE");
+    return v && *v && *v != '0';
+  }();
+  return on;
+}
+
+// solve_core with the single-reduction PCG (FEMGPU_CG_SINGLE_REDUCTION):
+// FP64 / FP32 run it directly; MIXED runs it in FP32 inside the same FP64
+// reliable-update loop as the standard recurrence.  At a residual
+// replacement the state is rebuilt from the FP64 residual: r = f - K x64,
+// u = M^-1 r, w = A u, and s = A p for the kept direction p (or p = s = 0
+// at a restart).
+static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, int block,
+                     const double *f64, double *x64, CgScalars *sc, CgScalars *sc2,
+                     femgpu_stats *S) {
+  const int64_t nd = ctx->nn * ctx->dim;
+  int st, done = 0;
+  Cg1Pos pos{0, 0};
+  if (o.precision == FEMGPU_FP64) {
+    femgpu::Cg1State<double> w;
+    if ((st = cg1_ws(ctx, "64", w))) return st;
+    const double *invD = static_cast<double *>(ctx->bufs["invD64"].p);
+    femgpu::launch_copy<double>(f64, w.r[0], nd, ctx->stream);
+    if ((st = run_cg1<double>(ctx, w, invD, o.tol_kind, o.tol, max_iter, block, 2, sc, &pos,
+                              &done)))
+      return st;
+    femgpu::launch_copy<double>(w.x[pos.q], x64, nd, ctx->stream);
+  } else if (o.precision == FEMGPU_FP32) {
+    femgpu::Cg1State<float> w;
+    if ((st = cg1_ws(ctx, "32", w))) return st;
+    const float *invD = static_cast<float *>(ctx->bufs["invD32"].p);
+    femgpu::launch_convert<double, float>(f64, w.r[0], nd, ctx->stream);
+    if ((st = run_cg1<float>(ctx, w, invD, o.tol_kind, o.tol, max_iter, block, 3, sc, &pos,
+                             &done)))
+      return st;
+    femgpu::launch_convert<float, double>(w.x[pos.q], x64, nd, ctx->stream);
+  } else if (o.precision == FEMGPU_MIXED) {
+    femgpu::Cg1State<float> w;
+    double *r64, *y64, *part64, *part_node, *dots;
+    if ((st = cg1_ws(ctx, "32", w)) || (st = dbuf(ctx, "ir_r", size_t(nd), &r64)) ||
+        (st = dbuf(ctx, "cg1dots", size_t(femgpu::CG1_DOT_PARTS), &dots)) ||
+        (st = dbuf(ctx, "ir_y", size_t(nd), &y64)) ||
+        (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part64)) ||
+        (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
+      return st;
+    const float *invD = static_cast<float *>(ctx->bufs["invD32"].p);
+    const double drop = o.inner_tol > 0 && o.inner_tol < 1 ? o.inner_tol : 0.1;
+    const int block_m = std::max(2, block + (block & 1));
+    femgpu::launch_fill<double>(x64, nd, 0.0, ctx->stream);
+    femgpu::launch_convert<double, float>(f64, w.r[0], nd, ctx->stream);
+    femgpu::launch_fill<float>(w.x[0], nd, 0.f, ctx->stream);
+    femgpu::launch_fill<float>(w.p[0], nd, 0.f, ctx->stream);
+    femgpu::launch_fill<float>(w.s[0], nd, 0.f, ctx->stream);
+    femgpu::launch_reset_scalars(sc, FEMGPU_TOL_REL_RES, o.tol, max_iter, ctx->stream);
+    femgpu::launch_cg1_init<float>(ctx->dm, w, invD, ctx->stream);
+    femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, nullptr, drop, ctx->stream);
+    HIPCHK(hipGetLastError());
+    if ((st = poll(ctx, sc))) return st;
+    const double ff = ctx->h_sc->rr0;
+    hipGraphExec_t exec;
+    if ((st = cg1_graph<float>(ctx, w, invD, block_m, 3, sc, &exec))) return st;
+    int status = FEMGPU_OK, replacements = 0;
+    double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
+    // The iterations pause on the device (done = 5) as soon as the FP32
+    // residual has dropped by `drop`: run further, the FP32 recurrences of
+    // s and w lose the accuracy the step-length formula relies on.
+    while (rel > o.tol) {
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0))) return st;
+      done = ctx->h_sc->done;
+      pos = cg1_pos(*ctx->h_sc);
+      if (trace_enabled())
+        std::fprintf(stderr, "femgpu   inner stop: r'r %.6e thresh %.6e pause %.6e rr0 %.6e it %d/%d\n",
+                     ctx->h_sc->rr_final, ctx->h_sc->thresh, ctx->h_sc->pause, ctx->h_sc->rr0,
+                     ctx->h_sc->it[0], ctx->h_sc->it[1]);
+      femgpu::launch_flush_x(ctx->dm, x64, w.x[pos.q], ctx->stream);
+      if (done != 0 && done != 1 && done != 5) {
+        status = done_status(done);
+        break;
+      }
+      // FP64 residual of the accumulated solution
+      femgpu::launch_reset_scalars(sc2, 1, 0.0, 0, ctx->stream);
+      femgpu::launch_matvec_patch<double>(ctx->dm, x64, y64, part64, ctx->stream);
+      femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
+      HIPCHK(hipGetLastError());
+      if ((st = poll(ctx, sc2))) return st;
+      rel = ff > 0 ? std::sqrt(ctx->h_sc->rr / ff) : 0.0;
+      if (trace_enabled())
+        std::fprintf(stderr, "femgpu mixed/single-reduction: %d iterations, stop %d, FP64 rel-res %.3e\n",
+                     pos.iters, done, rel);
+      if (rel <= o.tol) break;
+      if (pos.iters >= max_iter || replacements > 1000) {
+        status = FEMGPU_E_MAXIT;
+        break;
+      }
+      // Restart (p = s = 0) when the FP64 residual did not at least halve
+      // since the previous replacement: the FP32 directions have lost
+      // conjugacy.  Otherwise keep p (moved to index 0) with s = A p.
+      const bool restart = rel > 0.5 * rel_prev;
+      femgpu::launch_fill<float>(w.x[0], nd, 0.f, ctx->stream);
+      if (restart) {
+        femgpu::launch_fill<float>(w.p[0], nd, 0.f, ctx->stream);
+        femgpu::launch_fill<float>(w.s[0], nd, 0.f, ctx->stream);
+      } else {
+        if (pos.q) femgpu::launch_copy<float>(w.p[1], w.p[0], nd, ctx->stream);
+        femgpu::launch_matvec<float>(ctx->dm, w.p[0], w.s[0], w.slot[1], ctx->stream);
+      }
+      femgpu::launch_convert<double, float>(r64, w.r[0], nd, ctx->stream);
+      femgpu::launch_cg1_init<float>(ctx->dm, w, invD, ctx->stream);
+      if (!restart)
+        femgpu::launch_cg1_dots<float>(ctx->dm.n_active * ctx->dim, w.r[0], w.p[0], w.s[0], invD,
+                                       dots, ctx->stream);
+      femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, restart ? 2 : 1, pos.q, dots, drop,
+                             ctx->stream);
+      HIPCHK(hipGetLastError());
+      if (trace_enabled()) {
+        if ((st = poll(ctx, sc))) return st;
+        std::fprintf(stderr, "femgpu   replaced (%s): r'r %.6e g %.6e done %d it %d/%d\n",
+                     restart ? "restart" : "keep p", ctx->h_sc->rr_final, ctx->h_sc->g_final,
+                     ctx->h_sc->done, ctx->h_sc->it[0], ctx->h_sc->it[1]);
+      }
+      rel_prev = rel;
+      ++replacements;
+    }
+    S->iterations = pos.iters;
+    S->outer_iterations = replacements;
+    S->status = status;
+    return FEMGPU_OK;
+  } else {
+    return fail(ctx, FEMGPU_E_ARG, "unknown precision");
+  }
+  S->iterations = pos.iters;
+  S->gamma_ratio = ctx->h_sc->gamma0 > 0 ? ctx->h_sc->g_final / ctx->h_sc->gamma0 : 0.0;
+  S->status = done_status(done);
   return FEMGPU_OK;
 }
 
@@ -950,6 +1201,8 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     return fail(ctx, FEMGPU_E_ARG, "bad tolerance");
   if (o.precision == FEMGPU_MIXED && o.tol_kind != FEMGPU_TOL_REL_RES)
     return fail(ctx, FEMGPU_E_ARG, "MIXED precision needs FEMGPU_TOL_REL_RES");
+  if (o.recurrence != FEMGPU_CG_STANDARD && o.recurrence != FEMGPU_CG_SINGLE_REDUCTION)
+    return fail(ctx, FEMGPU_E_ARG, "unknown PCG recurrence");
   const int max_iter = o.max_iter > 0 ? o.max_iter : 1000000;
   const int block = o.check_every > 0 ? o.check_every : 32;
   if ((st = ensure_diag(ctx))) return st;
@@ -962,7 +1215,9 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
   femgpu_stats S{};
   int iters = 0, done = 0;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  if (o.precision == FEMGPU_FP64) {
+  if (o.recurrence == FEMGPU_CG_SINGLE_REDUCTION) {
+    if ((st = solve_cg1(ctx, o, max_iter, block, f64, x64, sc, sc2, &S))) return st;
+  } else if (o.precision == FEMGPU_FP64) {
     PcgWs<double> w;
     if ((st = pcg_ws(ctx, "c64", w))) return st;
     w.invD = static_cast<double *>(ctx->bufs["invD64"].p);

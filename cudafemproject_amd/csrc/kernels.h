@@ -22,6 +22,7 @@ struct DevMesh {
   const uint32_t *erec = nullptr;      // 3 words per element (pack_records)
   const int32_t *sh_gid = nullptr;
   const int32_t *sh_slot = nullptr;
+  const int4 *sh_info = nullptr;      // per sh entry: id, slot, node's first slot, end slot
   const uint16_t *node_end = nullptr;
   const int32_t *slot_start = nullptr;
   const uint8_t *fixmask = nullptr;   // per internal node: bit c = component c fixed
@@ -46,6 +47,8 @@ struct CgScalars {
   double eps;          // tol as given
   double g[2];         // gamma_j = r_j'z_j
   double b[2];         // beta_j
+  double al[2];        // alpha_j (single-reduction PCG, cg1.hip)
+  double pause;        // single-reduction PCG: stop with done = 5 once r'r < pause (0: never)
   double g_final;      // last gamma seen by the stop test
   double rr_final;     // last r'r seen by the stop test
   double pq;           // last p'q (diagnostic)
@@ -53,8 +56,10 @@ struct CgScalars {
   double scale;        // sqrt(rr) (refinement scaling)
   int it[2];           // updates done before iteration j
   int iter_final;      // updates done when the stop flag was set
+  int par_final;       // single-reduction PCG: parity of the launch that set the stop flag
   int kind;            // 0: gamma ratio, 1: relative residual
-  int done;            // 0 running, 1 converged, 2 non-finite, 3 breakdown, 4 max-iter
+  int done;            // 0 running, 1 converged, 2 non-finite, 3 breakdown, 4 max-iter,
+                       // 5 paused for a residual replacement (single-reduction PCG)
   int max_iter;
 };
 

@@ -13,8 +13,8 @@
 //  * k_pcg_update fuses the PCG vector updates and both dot products of an
 //    iteration (femfunc.cpp:59-111: addWeighted x3, divideElementwise,
 //    dotProduct x2) into one pass; the dot products reduce with wave64
-//    shuffles and a fixed-order last-block sum, and the scalars never leave
-//    the device.
+//    shuffles into fixed-order per-block partials that the next kernel
+//    reduces, and the scalars never leave the device.
 //  * The element-force math is the reference's K_e = B'DB * area * coeff
 //    (cpu_fem_utils.cpp:338-503) rewritten with the barycentric gradients
 //    g_a = c_a / det J (c_a: cofactor vectors), so that
@@ -28,6 +28,7 @@
 
 #include "element.h"
 #include "kernels.h"
+#include "patch_common.h"
 
 namespace femgpu {
 
@@ -36,25 +37,8 @@ namespace {
 constexpr int NT = 256;          // threads per block for every kernel
 constexpr int NODE_GRID_MAX = 2048;
 
-struct LdsLayout {
-  int off_c, off_x, off_res, off_red, total;
-};
-
-__host__ __device__ inline int al16(int v) { return (v + 31) & ~31; }  // 32-byte aligned regions
-
-// Patch kernel LDS: node coordinates and vector (pn x dim), one component of
-// the node-sorted element entries (ent, padded runs), reduction scratch.
-__host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ent) {
-  LdsLayout L;
-  L.off_c = 0;
-  L.off_x = al16(pn * dim * tb);
-  L.off_res = L.off_x + al16(pn * dim * tb);
-  L.off_red = L.off_res + al16(ent * tb);
-  L.total = L.off_red + 32 * 8;
-  return L;
-}
-
 using namespace elem;
+using namespace patch;
 
 // ---------------------------------------------------------------------------
 // Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step (stop test and beta
@@ -88,49 +72,12 @@ struct PatchArgs {
   int parity;         // MODE 1: j & 1
 };
 
-// Four consecutive values of T as one 16-byte (float) or 32-byte (double)
-// LDS access.
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<float> {
-  using type = float4;
-  __device__ static type zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
-  __device__ static float get(const type &v, int i) {
-    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
-  }
-};
-template <>
-struct Vec4<double> {
-  using type = double4;
-  __device__ static type zero() { return make_double4(0.0, 0.0, 0.0, 0.0); }
-  __device__ static double get(const type &v, int i) {
-    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
-  }
-};
-
-constexpr int EPT = 3;  // elements per thread: emax <= EPT * PT, patch nodes <= PT
-
-// Decode a 12-byte element record (pack_records): local node ids and the
-// positions of the element's corner entries in the patch's node-sorted list.
-__device__ __forceinline__ void decode_rec(uint3 r, int *l, int *q) {
-  l[0] = int(r.x & 1023u);
-  l[1] = int((r.x >> 10) & 1023u);
-  l[2] = int((r.x >> 20) & 1023u);
-  l[3] = int((r.x >> 30) | ((r.y & 255u) << 2));
-  q[0] = int((r.y >> 8) & 16383u);
-  q[1] = int((r.y >> 22) | ((r.z & 15u) << 10));
-  q[2] = int((r.z >> 4) & 16383u);
-  q[3] = int(r.z >> 18);
-}
-
 // One workgroup per patch, one patch node per thread (phases A and C), up to
 // EPT elements per thread (phase B).  The element forces stay in registers
 // and are assembled one component at a time through a single-component LDS
 // entry list, which keeps the LDS per element at npe words.
 template <typename T, int DIM, int MODE, int PT>
 __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
-  constexpr int NPE = DIM + 1, NDE = DIM * NPE;
   constexpr unsigned FMASK = (1u << DIM) - 1u;
   extern __shared__ __align__(16) unsigned char lds[];
   const int b = blockIdx.x;
@@ -201,14 +148,7 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
     esc[e] = a.escale && i < P.n_elem ? a.escale[P.elem_start + i] : T(1);
   }
 
-  // The entry list starts zeroed: the padding entries of the node runs
-  // (never written by an element) must read as zero.
-  {
-    using V4 = typename Vec4<T>::type;
-    V4 *r4 = reinterpret_cast<V4 *>(s_res);
-    const V4 z = Vec4<T>::zero();
-    for (int i = threadIdx.x; i < (a.ent >> 2); i += PT) r4[i] = z;
-  }
+  clear_entries<T, PT>(s_res, a.ent);
 
   // Phase A: this thread's patch node.  Constrained components enter the
   // element products as zero (their couplings are dropped); their rows are
@@ -237,7 +177,9 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
     } else if constexpr (MODE == 1) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) xin[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
-      if (own) {
+      // FP32: p is stored after the assembly (the phase barriers then do not
+      // wait for the writes); FP64 at once (registers)
+      if (sizeof(T) == 8 && own) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
       }
@@ -266,67 +208,10 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   }
   __syncthreads();
 
-  // Phase B: element forces in registers.
-  T F[EPT][NDE];
-  int q[EPT][4];
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int i = threadIdx.x + e * PT;
-    int l[4];
-    decode_rec(rec[e], l, q[e]);
-    if (i >= P.n_elem) continue;
-    T X[NDE], U[NDE];
-#pragma unroll
-    for (int n = 0; n < NPE; ++n)
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        X[n * DIM + c] = s_c[l[n] * DIM + c];
-        if constexpr (MODE != 2) U[n * DIM + c] = s_x[l[n] * DIM + c];
-      }
-    const T es = esc[e];
-    if constexpr (MODE == 2) {
-      elem_diag<T, DIM>(X, a.lam, a.mu, es, F[e]);
-    } else {
-      const T en = elem_force_free<T, DIM>(X, U, a.lam, a.mu, es, F[e]);
-      if constexpr (MODE == 1) acc_en += double(cK) * double(en);
-    }
-  }
-
-  // Phase C, one component at a time: entries to their node-sorted
-  // positions, then each node sums its contiguous run in order.
+  // Phases B and C: element forces, node-sorted LDS assembly, node sums.
   T acc[DIM];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) {
-#pragma unroll
-    for (int e = 0; e < EPT; ++e)
-      if (threadIdx.x + e * PT < P.n_elem)
-#pragma unroll
-        for (int n = 0; n < NPE; ++n) s_res[q[e][n]] = F[e][n * DIM + c];
-    __syncthreads();
-    // runs start and end on multiples of 4 entries: 16-byte reads, two in
-    // flight per step
-    using V4 = typename Vec4<T>::type;
-    const V4 *r4 = reinterpret_cast<const V4 *>(s_res);
-    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-    int j = b0 >> 2;
-    const int j1 = b1 >> 2;
-    for (; j + 2 <= j1; j += 2) {
-      const V4 u = r4[j], v = r4[j + 1];
-      s0 += Vec4<T>::get(u, 0) + Vec4<T>::get(v, 0);
-      s1 += Vec4<T>::get(u, 1) + Vec4<T>::get(v, 1);
-      s2 += Vec4<T>::get(u, 2) + Vec4<T>::get(v, 2);
-      s3 += Vec4<T>::get(u, 3) + Vec4<T>::get(v, 3);
-    }
-    if (j < j1) {
-      const V4 u = r4[j];
-      s0 += Vec4<T>::get(u, 0);
-      s1 += Vec4<T>::get(u, 1);
-      s2 += Vec4<T>::get(u, 2);
-      s3 += Vec4<T>::get(u, 3);
-    }
-    acc[c] = (s0 + s1) + (s2 + s3);
-    if (c + 1 < DIM) __syncthreads();
-  }
+  assemble<T, DIM, PT, MODE == 2>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0, b1, acc,
+                                  acc_en, double(cK));
 
   if (has_node) {
     // rows of constrained components: the assembled diagonal only
@@ -349,6 +234,10 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
         }
         a.y[gid * DIM + c] = v;
       }
+      if constexpr (MODE == 1 && sizeof(T) == 4) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
+      }
     } else {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = MODE == 1 ? cK * acc[c] : acc[c];
@@ -358,27 +247,6 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
   if constexpr (MODE == 1) {
     const double tot = block_sum<PT>(acc_en, s_red);
     if (threadIdx.x == 0) a.part[pid] = tot;
-  }
-}
-
-// acc += the slots [j0, j1) of an interface node, in slot order, two loads
-// in flight per step.
-template <typename T, int DIM>
-__device__ __forceinline__ void add_slots(const T *partial, int j0, int j1, T *acc) {
-  int j = j0;
-  for (; j + 2 <= j1; j += 2) {
-    T u[DIM], w[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      u[c] = partial[int64_t(j) * DIM + c];
-      w[c] = partial[int64_t(j + 1) * DIM + c];
-    }
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] = (acc[c] + u[c]) + w[c];
-  }
-  if (j < j1) {
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] += partial[int64_t(j) * DIM + c];
   }
 }
 
@@ -548,9 +416,10 @@ __global__ void k_reset_scalars(CgScalars *sc, int kind, double eps, int max_ite
     sc->gamma0 = sc->rr0 = sc->thresh = 0.0;
     sc->g[0] = sc->g[1] = 1.0;
     sc->b[0] = sc->b[1] = 0.0;
-    sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = 0.0;
+    sc->al[0] = sc->al[1] = 1.0;
+    sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = sc->pause = 0.0;
     sc->it[0] = sc->it[1] = -1;
-    sc->iter_final = 0;
+    sc->iter_final = sc->par_final = 0;
     sc->eps = eps;
     sc->kind = kind;
     sc->done = 0;

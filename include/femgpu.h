@@ -60,6 +60,18 @@ enum {
   FEMGPU_TOL_REL_RES = 1    /* ||f - K u||_2 <= tol * ||f||_2                */
 };
 
+/* PCG recurrences (same Jacobi-PCG, same iterates in exact arithmetic). */
+enum {
+  FEMGPU_CG_STANDARD = 0,         /* the reference's recurrence
+                                     (femfunc.cpp:59-111): two reductions
+                                     per iteration, two launches           */
+  FEMGPU_CG_SINGLE_REDUCTION = 1  /* Chronopoulos-Gear form: s = A p and
+                                     w = A M^-1 r carried as recurrences,
+                                     one reduction and one launch per
+                                     iteration; no counterpart in the
+                                     reference                              */
+};
+
 /* Load conventions for pressure faces and nodal loads. */
 enum {
   FEMGPU_LOADS_REFERENCE = 0, /* the reference's CPU path bit for bit in
@@ -159,6 +171,7 @@ typedef struct {
   double inner_tol;   /* MIXED: drop of the FP32 residual norm after which it
                          is replaced by the FP64 residual (default 0.1)       */
   int check_every;    /* iterations between host polls of the stop flag       */
+  int recurrence;     /* FEMGPU_CG_STANDARD (default) / _SINGLE_REDUCTION     */
 } femgpu_solve_opts;
 
 typedef struct {
