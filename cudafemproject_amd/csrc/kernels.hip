@@ -283,20 +283,82 @@ struct UpdArgs {
   double *prz, *prr;  // this iteration's r'z, r'r partials (NODE_PARTS each)
   int parity;
 };
-
 constexpr int UPT = 512;  // threads per block of the node-parallel PCG kernels
+
+// One node of the PCG update, loaded before alpha is known.
+template <typename T, int DIM>
+struct UpdNode {
+  T q[DIM], p[DIM], z[DIM], x[DIM], r[DIM], d[DIM];
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ void upd_load(const UpdArgs<T> &a, int64_t i, UpdNode<T, DIM> &n) {
+  const int64_t o = i * DIM;
+  if (i < a.n_owned) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      n.q[c] = a.q[o + c];
+      n.z[c] = T(0);
+    }
+  } else {
+    const int64_t s = i - a.n_owned;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      n.q[c] = T(0);
+      n.z[c] = a.z[o + c];
+    }
+    add_slots<T, DIM>(a.partial, a.slot_start[s], a.slot_start[s + 1], n.q);
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    n.p[c] = a.p[o + c];
+    n.x[c] = a.x[o + c];
+    n.r[c] = a.r[o + c];
+    n.d[c] = a.invD[o + c];
+  }
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNode<T, DIM> &n,
+                                           T alpha, T beta, double &rz, double &rr) {
+  const int64_t o = i * DIM;
+  if (i >= a.n_owned) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      n.p[c] = n.z[c] + beta * n.p[c];
+      a.p[o + c] = n.p[c];
+      if (a.sh.m) n.q[c] += a.sh.cM * a.sh.m[o + c] * n.p[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    a.x[o + c] = n.x[c] + alpha * n.p[c];
+    const T rv = n.r[c] - alpha * n.q[c];
+    a.r[o + c] = rv;
+    const T zv = n.d[c] * rv;
+    a.z[o + c] = zv;
+    rz += double(rv) * double(zv);
+    rr += double(rv) * double(rv);
+  }
+}
 
 // Fused PCG update (femfunc.cpp:66-81): alpha from the per-patch p'q
 // partials (every block reduces them in the same fixed order), finish the
 // interface nodes of q (and p), x += alpha p, r -= alpha q, z = invD r, and
-// per-block r'z, r'r partials for the next iteration's stop test.
+// per-block r'z, r'r partials for the next iteration's stop test.  A
+// thread's first node is loaded before the partials are reduced: those
+// loads do not depend on alpha, so their latency overlaps the reduction.
 template <typename T, int DIM>
 __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   __shared__ double s_red[2 * UPT / 64];
   CgScalars *sc = a.sc;
-  if (sc->done) return;
+  const int64_t stride = int64_t(gridDim.x) * UPT;
+  const int64_t i0 = int64_t(blockIdx.x) * UPT + threadIdx.x;
+  UpdNode<T, DIM> n0;
+  if (i0 < a.n_active) upd_load<T, DIM>(a, i0, n0);
   double pq = 0.0;
   for (int i = threadIdx.x; i < a.np; i += UPT) pq += a.ppq[i];
+  if (sc->done) return;
   pq = block_sum<UPT>(pq, s_red);
   const double gam = sc->g[a.parity];
   const double alpha_d = gam / pq;
@@ -310,38 +372,11 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq = pq;
   const T alpha = T(alpha_d), beta = T(sc->b[a.parity]);
   double rz = 0.0, rr = 0.0;
-  const int64_t stride = int64_t(gridDim.x) * UPT;
-  for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < a.n_active; i += stride) {
-    T qv[DIM], pv[DIM];
-    if (i < a.n_owned) {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        qv[c] = a.q[i * DIM + c];
-        pv[c] = a.p[i * DIM + c];
-      }
-    } else {
-      const int64_t s = i - a.n_owned;
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) qv[c] = T(0);
-      add_slots<T, DIM>(a.partial, a.slot_start[s], a.slot_start[s + 1], qv);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        pv[c] = a.z[i * DIM + c] + beta * a.p[i * DIM + c];
-        a.p[i * DIM + c] = pv[c];
-        if (a.sh.m) qv[c] += a.sh.cM * a.sh.m[i * DIM + c] * pv[c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      const int64_t o = i * DIM + c;
-      a.x[o] = a.x[o] + alpha * pv[c];
-      const T rv = a.r[o] - alpha * qv[c];
-      a.r[o] = rv;
-      const T zv = a.invD[o] * rv;
-      a.z[o] = zv;
-      rz += double(rv) * double(zv);
-      rr += double(rv) * double(rv);
-    }
+  if (i0 < a.n_active) upd_finish<T, DIM>(a, i0, n0, alpha, beta, rz, rr);
+  for (int64_t i = i0 + stride; i < a.n_active; i += stride) {
+    UpdNode<T, DIM> n;
+    upd_load<T, DIM>(a, i, n);
+    upd_finish<T, DIM>(a, i, n, alpha, beta, rz, rr);
   }
   block_sum2<UPT>(rz, rr, s_red);
   if (threadIdx.x == 0) {
@@ -349,7 +384,6 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
     a.prr[blockIdx.x] = rr;
   }
 }
-
 // PCG initialisation: z = invD r and the r'z, r'r partials that the first
 // fused iteration (parity 0) reads from slot 1.
 template <typename T>
