@@ -699,7 +699,10 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   const int npe = dim + 1;
   // one workgroup thread per patch node and at most EPT = 3 elements per
   // thread: patches of <= 3 * threads elements and <= threads nodes
-  const int threads = ctx->threads_req ? ctx->threads_req : 1024;
+  // Default 256 threads (768-element patches): on gfx950 the fastest PCG
+  // iteration in FP64 (no register spills, three workgroups per CU) and
+  // within a few percent of the fastest in FP32 among 256/512/1024.
+  const int threads = ctx->threads_req ? ctx->threads_req : 256;
   int emax = ctx->emax > 0 ? ctx->emax : 3 * threads;
   if (emax > 3 * threads) return fail(ctx, FEMGPU_E_ARG, "elements per patch above 3 x patch threads");
   // The FP64 patch kernels must fit a workgroup's LDS (160 KiB on gfx950):
@@ -831,7 +834,7 @@ int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads) {
     return fail(ctx, FEMGPU_E_ARG, "elems_per_patch out of range");
   if (patch_threads != 0 && patch_threads != 256 && patch_threads != 512 && patch_threads != 1024)
     return fail(ctx, FEMGPU_E_ARG, "patch_threads must be 256, 512 or 1024");
-  const int threads = patch_threads ? patch_threads : 1024;
+  const int threads = patch_threads ? patch_threads : 256;
   if (elems_per_patch > 3 * threads)
     return fail(ctx, FEMGPU_E_ARG, "elems_per_patch above 3 x patch_threads");
   // takes effect at the next femgpu_set_mesh (the plan depends on both)

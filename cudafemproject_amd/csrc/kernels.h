@@ -16,7 +16,7 @@ struct DevMesh {
   int emax = 0;            // elements per patch bound
   int ent = 0;             // LDS entry capacity (longest padded entry list, multiple of 4)
   int pn = 0;              // LDS node capacity
-  int patch_threads = 1024; // threads per patch workgroup (256, 512 or 1024)
+  int patch_threads = 256;  // threads per patch workgroup (256, 512 or 1024)
   int64_t nn = 0, ne = 0, n_owned = 0, n_iface = 0, n_active = 0;
   const PatchInfo *patches = nullptr;
   const uint32_t *erec = nullptr;      // 3 words per element (pack_records)

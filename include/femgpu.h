@@ -117,7 +117,7 @@ int femgpu_sync(femgpu_ctx *ctx);
 int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nodes,
                     int64_t n_elems, const int32_t *elems);
 /* Launch tuning, applied at the next femgpu_set_mesh: threads per patch
- * workgroup (256, 512 or 1024; 0 = 1024), which is also the node limit of a
+ * workgroup (256, 512 or 1024; 0 = 256), which is also the node limit of a
  * patch, and elements per patch (<= 3 x threads; 0 = 3 x threads).
  * No counterpart in the reference (its kernels use 256 threads and one
  * thread per element, fem_utils_kernels.cu:3). */
