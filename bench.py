@@ -362,10 +362,10 @@ def main():
     log(f"setup {time.perf_counter() - t0:.2f} s: {pb.n_elems} elems, {pb.n_nodes} nodes, "
         f"{pb.n_dof} dof")
 
-    def run(prec, k):
+    def run(prec, k, rec=F.CG_STANDARD):
         out = []
         for _ in range(k):
-            u, st = ctx.solve(prec, tol=1e-8, tol_kind=F.TOL_REL_RES)
+            u, st = ctx.solve(prec, tol=1e-8, tol_kind=F.TOL_REL_RES, recurrence=rec)
             out.append((u, st))
         return out
 
@@ -387,6 +387,17 @@ def main():
     log(f"fp64: {t64 * 1e3:.2f} ms, {s64['iterations']} it, rel_res {s64['rel_res']:.2e}")
     log(f"mixed: {tmx * 1e3:.2f} ms, {smx['iterations']} it ({smx['outer_iterations']} outer), "
         f"rel_res {smx['rel_res']:.2e}, |u_mixed-u_fp64|/|u_fp64| {err:.2e}")
+    # the same solves with the single-reduction recurrence (one launch per
+    # iteration); the FP64 baseline of the speedup is the faster of the two
+    run(F.FP64, 1, F.CG_SINGLE_REDUCTION)
+    run(F.MIXED, 1, F.CG_SINGLE_REDUCTION)
+    r64s = run(F.FP64, args.steps, F.CG_SINGLE_REDUCTION)
+    rmxs = run(F.MIXED, args.steps, F.CG_SINGLE_REDUCTION)
+    t64s = np.mean([s["solve_ms"] for _, s in r64s]) / 1e3
+    tmxs = np.mean([s["solve_ms"] for _, s in rmxs]) / 1e3
+    log(f"single-reduction recurrence: fp64 {t64s * 1e3:.2f} ms ({r64s[-1][1]['iterations']} it), "
+        f"mixed {tmxs * 1e3:.2f} ms ({rmxs[-1][1]['iterations']} it)")
+    t64_best = min(t64, t64s)
 
     mv32 = ctx.time_matvec(F.FP32, args.matvec_reps) / 1e3
     mv64 = ctx.time_matvec(F.FP64, args.matvec_reps) / 1e3
@@ -416,9 +427,14 @@ def main():
         "config": {"workload": f"cube {args.n}^3 hexes x 6 Kuhn tets, z=0 clamped, p=1e3 on z=1, "
                                "static PCG to rel-res 1e-8",
                    "n_elems": pb.n_elems, "n_nodes": pb.n_nodes, "n_dof": pb.n_dof,
-                   "tol": 1e-8, "solver": "mixed (fp32 pcg in fp64 refinement)"},
+                   "tol": 1e-8, "solver": "mixed (fp32 pcg in fp64 refinement)",
+                   "patch_tuning": {"threads": 256, "elements": 768}},
         "fp64_time_to_solve_s": float(t64),
-        "speedup_mixed_vs_fp64": float(t64 / tmx),
+        "speedup_mixed_vs_fp64": float(t64_best / tmx),
+        "fp64_single_reduction_time_to_solve_s": float(t64s),
+        "mixed_single_reduction_time_to_solve_s": float(tmxs),
+        "iterations_single_reduction_fp64": r64s[-1][1]["iterations"],
+        "iterations_single_reduction_mixed": rmxs[-1][1]["iterations"],
         "mixed_vs_fp64_rel_displacement_diff": err,
         "iterations_fp64": s64["iterations"], "iterations_mixed": smx["iterations"],
         "outer_iterations_mixed": smx["outer_iterations"],
