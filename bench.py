@@ -232,7 +232,8 @@ def bench_cantilever(args, F):
 
 
 def bench_batch(args, F):
-    """512 independent plates x 4 load cases in one persistent launch."""
+    """512 independent plates x 4 load cases (lockstep schedule; the
+    persistent one-workgroup-per-instance schedule is timed beside it)."""
     from cudafemproject_amd.problem import load_batch, plate_batch
     insts = plate_batch(512, seed=2024)
     ctx = F.Context(0)
@@ -240,12 +241,16 @@ def bench_batch(args, F):
     rhs = [b.f for b in insts]
     n_sys = len(insts) * 4
     res = {}
-    for name, prec in (("mixed", F.MIXED), ("fp64", F.FP64)):
+    runs = [("mixed", F.MIXED, F.BATCH_LOCKSTEP), ("fp64", F.FP64, F.BATCH_LOCKSTEP)]
+    if not args.no_persistent:
+        runs += [("mixed_persistent", F.MIXED, F.BATCH_PERSISTENT),
+                 ("fp64_persistent", F.FP64, F.BATCH_PERSISTENT)]
+    for name, prec, sched in runs:
         for _ in range(args.warmup):
-            ctx.batch_solve(rhs, prec, tol=1e-8)
+            ctx.batch_solve(rhs, prec, tol=1e-8, schedule=sched)
         ms = []
         for _ in range(args.steps):
-            us, st, m = ctx.batch_solve(rhs, prec, tol=1e-8)
+            us, st, m = ctx.batch_solve(rhs, prec, tol=1e-8, schedule=sched)
             ms.append(m)
         res[name] = (float(np.mean(ms)), us, st)
         log(f"{name}: {np.mean(ms):.2f} ms for {n_sys} systems, iterations max "
@@ -277,11 +282,14 @@ def bench_batch(args, F):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "512 2D plates 1.0x0.5, nx~U{16..160}, ny~U{16..80} (seed 2024), "
                                "E~LogNormal(ln 2e7, 0.5), nu~U[0.2,0.35], left edge fixed, "
-                               "4 load cases (x/y traction, shear, body force), one launch",
+                               "4 load cases (x/y traction, shear, body force)",
                    "n_instances": len(insts), "n_systems": n_sys,
                    "n_elems": int(sum(len(b.elems) for b in insts)),
                    "n_dof": int(sum(2 * len(b.nodes) for b in insts)), "tol": 1e-8},
         "fp64_s": t64 / 1e3, "speedup_mixed_vs_fp64": t64 / tmx,
+        "schedule": "lockstep (every system over the whole GPU, graph replays)",
+        "persistent_schedule_s": ({k: res[k][0] / 1e3 for k in ("mixed_persistent", "fp64_persistent")}
+                                  if "mixed_persistent" in res else None),
         "systems_per_s_mixed": n_sys / (tmx * 1e-3),
         "iterations_max_mixed": int(smx["iterations"].max()),
         "iterations_mean_mixed": float(smx["iterations"].mean()),
@@ -289,7 +297,7 @@ def bench_batch(args, F):
         "mixed_vs_fp64_max_rel_diff": diff, "vs_single_solver_max_rel_diff": max(chk),
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_batch_pcg<float,2,4> (whole batch, one launch)",
+                     "kernel": "k_bl_patch<float,2,4,1> + k_bl_update<float,2,4> (all iterations)",
                      "bytes_per_launch": tot},
         "cpu_baseline": None,
     }
@@ -330,6 +338,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the FP32 matvec + copy sequence profiled for HBM traffic")
     ap.add_argument("--fd-elems", type=int, default=64)
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="batch workload: skip the persistent-schedule comparison")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)

@@ -4,7 +4,7 @@ Drop-in for the solver path of moarseniy/CudaFEMproject (femfunc: static
 Jacobi-PCG, GN22 explicit/implicit dynamics) built on hand-written HIP
 kernels for gfx950 behind the C-ABI of include/femgpu.h.
 """
-from ._lib import (CG_SINGLE_REDUCTION, CG_STANDARD, FP32, FP64, MIXED, IMPLICIT_PHYSICAL, IMPLICIT_REFERENCE, LOADS_PHYSICAL,
+from ._lib import (BATCH_LOCKSTEP, BATCH_PERSISTENT, CG_SINGLE_REDUCTION, CG_STANDARD, FP32, FP64, MIXED, IMPLICIT_PHYSICAL, IMPLICIT_REFERENCE, LOADS_PHYSICAL,
                    LOADS_REFERENCE, MASS_PHYSICAL, MASS_REFERENCE, TOL_REF_GAMMA, TOL_REL_RES,
                    RECV_ACCELERATION, RECV_DISPLACEMENT, RECV_VELOCITY, WAVELET_BERLAGE,
                    WAVELET_NONE, WAVELET_RICKER, Context, FemGpuError,
@@ -15,4 +15,4 @@ __all__ = ["Context", "FemGpuError", "Problem", "load_library", "FP32", "FP64", 
            "TOL_REF_GAMMA", "TOL_REL_RES", "LOADS_REFERENCE", "LOADS_PHYSICAL",
            "MASS_REFERENCE", "MASS_PHYSICAL", "WAVELET_NONE", "WAVELET_RICKER",
            "WAVELET_BERLAGE", "IMPLICIT_REFERENCE", "IMPLICIT_PHYSICAL", "CG_STANDARD",
-           "CG_SINGLE_REDUCTION", "RECV_DISPLACEMENT", "RECV_VELOCITY", "RECV_ACCELERATION"]
+           "CG_SINGLE_REDUCTION", "BATCH_LOCKSTEP", "BATCH_PERSISTENT", "RECV_DISPLACEMENT", "RECV_VELOCITY", "RECV_ACCELERATION"]

@@ -25,6 +25,7 @@ STATUS_NAMES = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_NONFINITE", -4: "E_MAX
 FP64, FP32, MIXED = 0, 1, 2
 TOL_REF_GAMMA, TOL_REL_RES = 0, 1
 CG_STANDARD, CG_SINGLE_REDUCTION = 0, 1
+BATCH_LOCKSTEP, BATCH_PERSISTENT = 0, 1
 LOADS_REFERENCE, LOADS_PHYSICAL = 0, 1
 MASS_REFERENCE, MASS_PHYSICAL = 0, 1
 WAVELET_NONE, WAVELET_RICKER, WAVELET_BERLAGE = 0, 1, 2
@@ -35,7 +36,7 @@ RECV_DISPLACEMENT, RECV_VELOCITY, RECV_ACCELERATION = 0, 1, 2
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("tol_kind", C.c_int), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("inner_tol", C.c_double), ("check_every", C.c_int),
-                ("recurrence", C.c_int)]
+                ("recurrence", C.c_int), ("batch_schedule", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -371,9 +372,12 @@ class Context:
                                                           _ip(codes)), "batch_set_constraints")
 
     def batch_solve(self, rhs, precision: int = MIXED, tol: float = 1e-8, max_iter: int = 0,
-                    inner_tol: float = 0.1, raise_on_error: bool = True):
+                    inner_tol: float = 0.1, raise_on_error: bool = True,
+                    schedule: int = BATCH_LOCKSTEP):
         """rhs: list of (n_i, n_rhs, dim) arrays.  Returns (list of u arrays of
-        the same shapes, stats as a structured array (n_inst, n_rhs), device ms)."""
+        the same shapes, stats as a structured array (n_inst, n_rhs), device ms).
+        schedule: BATCH_LOCKSTEP (every instance over the whole GPU, graph
+        replays) or BATCH_PERSISTENT (one workgroup per instance, one launch)."""
         dim, noff = self._batch
         nr = int(np.asarray(rhs[0]).shape[1])
         f = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, nr, dim)
@@ -382,6 +386,7 @@ class Context:
         u = np.zeros_like(f)
         st = (BatchStat * (len(rhs) * nr))()
         o = self._opts(precision, tol, TOL_REL_RES, max_iter, inner_tol, 32)
+        o.batch_schedule = schedule
         ms = C.c_double(0)
         code = self.lib.femgpu_batch_solve(self.h, nr, _dp(f), C.byref(o), _dp(u), st, C.byref(ms))
         if raise_on_error:

@@ -18,6 +18,7 @@
 
 #include "femgpu.h"
 #include "batch.h"
+#include "batch_ls.h"
 #include "cg1.h"
 #include "femgpu_internal.h"
 #include "kernels.h"
@@ -643,6 +644,7 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   HIPCHK(hipEventCreate(&ctx->ev1));
   HIPCHK(femgpu::prepare_kernels());
   HIPCHK(femgpu::prepare_cg1_kernels());
+  HIPCHK(femgpu::prepare_bl_kernels());
   *out = c.release();
   return FEMGPU_OK;
 }
@@ -1048,6 +1050,7 @@ int femgpu_default_solve_opts(femgpu_solve_opts *o) {
   o->inner_tol = 0.1;
   o->check_every = 32;
   o->recurrence = FEMGPU_CG_STANDARD;
+  o->batch_schedule = FEMGPU_BATCH_LOCKSTEP;
   return FEMGPU_OK;
 }
 
@@ -1704,6 +1707,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
         if (relax) {
           double vm = 0.0;
           if ((st = vmax(&vm))) {
+            (void)hipStreamSynchronize(ctx->stream);
             (void)hipGraphExecDestroy(exec);
             return st;
           }

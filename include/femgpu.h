@@ -60,6 +60,15 @@ enum {
   FEMGPU_TOL_REL_RES = 1    /* ||f - K u||_2 <= tol * ||f||_2                */
 };
 
+/* Schedules of femgpu_batch_solve (no counterpart in the reference). */
+enum {
+  FEMGPU_BATCH_LOCKSTEP = 0,   /* every iteration of every system in two
+                                  grid-wide launches (graph replays): each
+                                  instance spread over the whole GPU        */
+  FEMGPU_BATCH_PERSISTENT = 1  /* one workgroup runs one instance's whole
+                                  solve; the batch is one launch             */
+};
+
 /* PCG recurrences (same Jacobi-PCG, same iterates in exact arithmetic). */
 enum {
   FEMGPU_CG_STANDARD = 0,         /* the reference's recurrence
@@ -172,6 +181,8 @@ typedef struct {
                          is replaced by the FP64 residual (default 0.1)       */
   int check_every;    /* iterations between host polls of the stop flag       */
   int recurrence;     /* FEMGPU_CG_STANDARD (default) / _SINGLE_REDUCTION     */
+  int batch_schedule; /* femgpu_batch_solve: FEMGPU_BATCH_LOCKSTEP (default) /
+                         FEMGPU_BATCH_PERSISTENT                              */
 } femgpu_solve_opts;
 
 typedef struct {

@@ -13,6 +13,9 @@ from helpers import dense_K
 
 pytestmark = pytest.mark.gpu
 
+SCHEDULES = pytest.mark.parametrize("sched", [F.BATCH_LOCKSTEP, F.BATCH_PERSISTENT],
+                                    ids=["lockstep", "persistent"])
+
 
 def rel(a, b):
     a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
@@ -34,11 +37,12 @@ def dense_solution(b):
 
 
 @pytest.mark.parametrize("precision,tol,bound", [(F.FP64, 1e-12, 1e-9), (F.MIXED, 1e-10, 1e-7)])
-def test_batch_vs_dense(ctx, precision, tol, bound):
+@SCHEDULES
+def test_batch_vs_dense(ctx, precision, tol, bound, sched):
     insts = small_plates(9, seed=3)
     print(f"[batch vs dense] prec={precision} n_inst={len(insts)}", flush=True)
     load_batch(ctx, insts)
-    us, st, ms = ctx.batch_solve([b.f for b in insts], precision=precision, tol=tol)
+    us, st, ms = ctx.batch_solve([b.f for b in insts], precision=precision, tol=tol, schedule=sched)
     assert np.all(st["status"] == 0)
     assert np.all(st["rel_res"] <= tol)
     assert np.all(st["iterations"] > 0)
@@ -47,23 +51,25 @@ def test_batch_vs_dense(ctx, precision, tol, bound):
 
 
 @pytest.mark.parametrize("nr", [1, 2, 3])
-def test_batch_rhs_counts(ctx, nr):
+@SCHEDULES
+def test_batch_rhs_counts(ctx, nr, sched):
     insts = small_plates(5, seed=7, nr=nr)
     load_batch(ctx, insts)
-    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-12)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-12, schedule=sched)
     assert st.shape == (5, nr) and np.all(st["status"] == 0)
     for b, u in zip(insts, us):
         assert u.shape == b.f.shape
         assert rel(u, dense_solution(b)) < 1e-9
 
 
-def test_batch_deterministic_and_zero_rhs(ctx):
+@SCHEDULES
+def test_batch_deterministic_and_zero_rhs(ctx, sched):
     insts = small_plates(6, seed=11)
     insts[2].f[:] = 0.0                     # an instance with no load at all
     insts[4].f[:, 1] = 0.0                  # one load case of another
     load_batch(ctx, insts)
-    u1, s1, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-9)
-    u2, s2, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-9)
+    u1, s1, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-9, schedule=sched)
+    u2, s2, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-9, schedule=sched)
     for a, b in zip(u1, u2):
         assert np.array_equal(a, b)
     assert np.array_equal(s1, s2)
@@ -72,7 +78,8 @@ def test_batch_deterministic_and_zero_rhs(ctx):
     assert np.all(s1["status"] == 0)
 
 
-def test_batch_3d(ctx):
+@SCHEDULES
+def test_batch_3d(ctx, sched):
     insts = []
     rng = np.random.default_rng(5)
     for i in range(4):
@@ -85,30 +92,49 @@ def test_batch_3d(ctx):
                                    nu=float(rng.uniform(0.2, 0.35)), constr_node=fixed,
                                    constr_code=np.full(len(fixed), 7, np.int32), f=f))
     load_batch(ctx, insts)
-    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-12)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-12, schedule=sched)
     assert np.all(st["status"] == 0)
     for b, u in zip(insts, us):
         assert rel(u, dense_solution(b)) < 1e-9
 
 
-def test_batch_max_iter_status(ctx):
+@SCHEDULES
+def test_batch_max_iter_status(ctx, sched):
     insts = small_plates(3, seed=13)
     load_batch(ctx, insts)
     us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.FP64, tol=1e-14, max_iter=3,
-                                raise_on_error=False)
+                                raise_on_error=False, schedule=sched)
     assert np.all(st["status"] == E_MAXIT)
     assert np.all(st["iterations"] == 3)
 
 
-def test_batch_matches_single_instance_path(ctx):
+@SCHEDULES
+def test_batch_matches_single_instance_path(ctx, sched):
     """Plates of the benchmark's size range against the single-problem solver
     (femgpu_solve_rhs) on the same instance and load case."""
     insts = plate_batch(6, seed=21, nx_range=(16, 60), ny_range=(16, 40))
     load_batch(ctx, insts)
-    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-10)
+    us, st, _ = ctx.batch_solve([b.f for b in insts], precision=F.MIXED, tol=1e-10, schedule=sched)
     assert np.all(st["status"] == 0)
     for b, u in zip(insts, us):
         ctx.set_problem(b.problem())
         for r in range(4):
             us1, s1 = ctx.solve_rhs(b.f[:, r], F.FP64, tol=1e-12)
             assert rel(u[:, r], us1) < 1e-8
+
+
+def test_batch_schedules_agree(ctx):
+    """Both schedules on the same batch (instances of several patches)."""
+    insts = plate_batch(12, seed=31, nx_range=(8, 70), ny_range=(8, 40))
+    load_batch(ctx, insts)
+    for prec, tol, bound in ((F.FP64, 1e-11, 1e-9), (F.MIXED, 1e-9, 1e-7)):
+        ua, sa, ma = ctx.batch_solve([b.f for b in insts], precision=prec, tol=tol,
+                                     schedule=F.BATCH_LOCKSTEP)
+        ub, sb, mb = ctx.batch_solve([b.f for b in insts], precision=prec, tol=tol,
+                                     schedule=F.BATCH_PERSISTENT)
+        print(f"prec {prec}: lockstep {ma:.2f} ms (max it {sa['iterations'].max()}), "
+              f"persistent {mb:.2f} ms (max it {sb['iterations'].max()})", flush=True)
+        assert np.all(sa["status"] == 0) and np.all(sb["status"] == 0)
+        assert np.all(sa["rel_res"] <= tol) and np.all(sb["rel_res"] <= tol)
+        for a, b in zip(ua, ub):
+            assert rel(a, b) < bound
