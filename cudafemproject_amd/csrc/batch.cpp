@@ -155,8 +155,8 @@ std::string build_lockstep(BatchState *b, hipStream_t s, const int64_t *node_off
       iface.insert(iface.end(), {N0 + int32_t(Q.n_owned + t), S0 + Q.slot_start[t],
                                  S0 + Q.slot_start[t + 1], 0});
     I.u0 = int32_t(blocks.size());
-    for (int32_t n = 0; n < I.n_active; n += BL_THREADS)
-      blocks.push_back({i, N0 + n, N0 + std::min(I.n_active, n + BL_THREADS), 0});
+    for (int32_t n = 0; n < I.n_active; n += BL_UPD_NODES)
+      blocks.push_back({i, N0 + n, N0 + std::min(I.n_active, n + BL_UPD_NODES), 0});
     I.u1 = int32_t(blocks.size());
     const double E = 2e7, nu = 0.3;
     I.mu = E / (2.0 * (1.0 + nu));

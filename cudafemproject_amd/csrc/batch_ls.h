@@ -21,7 +21,8 @@
 
 namespace femgpu {
 
-constexpr int BL_THREADS = 256;  // patch workgroup (= patch node limit) and update block
+constexpr int BL_THREADS = 256;    // patch workgroup (= patch node limit)
+constexpr int BL_UPD_NODES = 64;   // nodes per node-parallel block (64 x NR threads)
 
 struct BLInst {
   int32_t node_off, n_owned, n_active, n_nodes;
@@ -79,19 +80,19 @@ template <typename T>
 void bl_inv_diag(int64_t n, const double *diag, T *invD, hipStream_t s);
 // per (instance, rhs): state 1 (replacement requested), counters reset
 void bl_init_state(int n_state, BLState *st, hipStream_t s);
-// One PCG iteration (patch step + node update) at launch parity `parity`.
+// One PCG iteration (patch step + node update) at launch parity `parity`;
+// the solution x64 is accumulated in FP64.
 template <typename T>
 void bl_iteration(int dim, int nr, const BLArgs<T> &a, const T *invD, T *z, T *p, T *q, T *r,
                   T *slots, double *x64, double *ppq, double *pnode, int parity, hipStream_t s);
 // Residual replacement of the systems with state 1: y64 = K x64 (flagged
-// instances), r = f - y64, z = invD r, partials for the next iteration;
-// then state 1 -> 2.  count: number of unfinished systems.
+// instances), r = f - y64, z = invD r, partials for the next iteration; then
+// state 1 -> 2.  count: unfinished systems.
 template <typename T>
 void bl_replace(int dim, int nr, const BLArgs<T> &a, const BLArgs<double> &a64, const double *f,
                 const double *x64, double *y64, double *slots64, const T *invD, T *r, T *z,
                 double *pnode, int n_state, int *count, hipStream_t s);
-// Final FP64 residual of every system: rel_res into the state's rr field
-// as r'r / f'f.
+// Final FP64 residual of every system: r'r into the state's rr field.
 void bl_final(int dim, int nr, const BLArgs<double> &a64, const double *f, const double *x64,
               double *y64, double *slots64, double *pnode, int n_inst, hipStream_t s);
 

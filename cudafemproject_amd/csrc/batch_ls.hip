@@ -338,21 +338,51 @@ __global__ __launch_bounds__(PT) void k_bl_patch(BLPatch<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Node-parallel kernels: one thread per (node, right-hand side) of a block of
+// BL_UPD_NODES nodes, so that a wave reads contiguous [node][rhs][dim] data.
+
+// Per-rhs block totals of a (thread rhs = lane % NR): xor butterfly over the
+// lanes of one rhs, then the waves in order.  sh: 2 * NR * NR doubles.
+// Totals valid in threads < NR (thread v: rhs v).
+template <int NR>
+__device__ __forceinline__ void rhs_sums(double &a, double &b, double *sh) {
+#pragma unroll
+  for (int o = 32; o >= NR; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l < NR) {
+    sh[(w * NR + l) * 2] = a;
+    sh[(w * NR + l) * 2 + 1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < NR; ++i) {  // NR waves per block
+      sa += sh[(i * NR + threadIdx.x) * 2];
+      sb += sh[(i * NR + threadIdx.x) * 2 + 1];
+    }
+    a = sa;
+    b = sb;
+  }
+}
+
 // Node update of one PCG iteration.
 template <typename T>
 struct BLUpdate {
   BLArgs<T> m;
   const T *q, *slots, *invD;
   T *p, *r, *z;
-  double *x;
+  double *x;           // solution, accumulated in FP64
   const double *ppq;
   double *prz, *prr;   // this iteration's partials [nblk][NR]
   int parity;
 };
 
 template <typename T, int DIM, int NR>
-__global__ __launch_bounds__(PT) void k_bl_update(BLUpdate<T> a) {
-  __shared__ double s_red[2 * NR * PT / 64];
+__global__ __launch_bounds__(64 * NR) void k_bl_update(BLUpdate<T> a) {
+  __shared__ double s_red[2 * NR * NR];
   __shared__ double s_al[NR], s_bt[NR];
   __shared__ int s_act[NR];
   const BLBlock B = a.m.blocks[blockIdx.x];
@@ -383,70 +413,57 @@ __global__ __launch_bounds__(PT) void k_bl_update(BLUpdate<T> a) {
     }
   }
   __syncthreads();
-  double part[2 * NR];
-#pragma unroll
-  for (int v = 0; v < 2 * NR; ++v) part[v] = 0.0;
-  const int64_t g = B.n0 + threadIdx.x;
-  if (g < B.n1) {
+  const int v = threadIdx.x % NR;
+  const int64_t g = B.n0 + threadIdx.x / NR;
+  double rz = 0.0, rr = 0.0;
+  if (g < B.n1 && s_act[v]) {
     const int64_t loc = g - I.node_off;
-    int s0 = 0, s1 = 0;
-    const bool own = loc < I.n_owned;
-    if (!own) {
-      const int4 fi = a.m.iface[I.if0 + (loc - I.n_owned)];
-      s0 = fi.y;
-      s1 = fi.z;
-    }
-#pragma unroll
-    for (int v = 0; v < NR; ++v) {
-      if (!s_act[v]) continue;
-      const int64_t o = (g * NR + v) * DIM;
-      T qv[DIM], pv[DIM];
-      if (own) {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-          qv[c] = a.q[o + c];
-          pv[c] = a.p[o + c];
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) qv[c] = T(0);
-        for (int j = s0; j < s1; ++j)
-#pragma unroll
-          for (int c = 0; c < DIM; ++c) qv[c] += a.slots[(int64_t(j) * NR + v) * DIM + c];
-        const T bt = T(s_bt[v]);
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-          pv[c] = a.z[o + c] + bt * a.p[o + c];
-          a.p[o + c] = pv[c];
-        }
-      }
-      const T al = T(s_al[v]);
+    const int64_t o = (g * NR + v) * DIM;
+    T qv[DIM], pv[DIM];
+    if (loc < I.n_owned) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        a.x[o + c] += s_al[v] * double(pv[c]);
-        const T rv = a.r[o + c] - al * qv[c];
-        a.r[o + c] = rv;
-        const T zv = a.invD[g * DIM + c] * rv;
-        a.z[o + c] = zv;
-        part[v] += double(rv) * double(zv);
-        part[NR + v] += double(rv) * double(rv);
+        qv[c] = a.q[o + c];
+        pv[c] = a.p[o + c];
+      }
+    } else {
+      const int4 fi = a.m.iface[I.if0 + (loc - I.n_owned)];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) qv[c] = T(0);
+      for (int j = fi.y; j < fi.z; ++j)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) qv[c] += a.slots[(int64_t(j) * NR + v) * DIM + c];
+      const T bt = T(s_bt[v]);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        pv[c] = a.z[o + c] + bt * a.p[o + c];
+        a.p[o + c] = pv[c];
       }
     }
-  }
-  block_sums<2 * NR>(part, s_red);
-  if (threadIdx.x == 0)
+    const T al = T(s_al[v]);
 #pragma unroll
-    for (int v = 0; v < NR; ++v) {
-      a.prz[int64_t(blockIdx.x) * NR + v] = part[v];
-      a.prr[int64_t(blockIdx.x) * NR + v] = part[NR + v];
+    for (int c = 0; c < DIM; ++c) {
+      a.x[o + c] += s_al[v] * double(pv[c]);
+      const T rv = a.r[o + c] - al * qv[c];
+      a.r[o + c] = rv;
+      const T zv = a.invD[g * DIM + c] * rv;
+      a.z[o + c] = zv;
+      rz += double(rv) * double(zv);
+      rr += double(rv) * double(rv);
     }
+  }
+  rhs_sums<NR>(rz, rr, s_red);
+  if (threadIdx.x < NR) {
+    a.prz[int64_t(blockIdx.x) * NR + threadIdx.x] = rz;
+    a.prr[int64_t(blockIdx.x) * NR + threadIdx.x] = rr;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Residual replacement of the systems with state 1 (all systems at the
-// start): r64 = f - y64 (interface nodes from the FP64 slots), r = r64,
-// z = D^-1 r, and the partials the next iteration (parity 0) reads: r'z and
-// the FP64 r'r.  FINAL: only the FP64 r'r of every system, into rr.
+// start): r64 = f - y64 with y64 = K x (interface nodes from the FP64
+// slots), r = r64, z = D^-1 r, and the partials the next iteration (parity
+// 0) reads: r'z and the FP64 r'r.  FINAL: every system, only the FP64 r'r.
 template <typename T>
 struct BLReplace {
   BLArgs<T> m;
@@ -457,67 +474,51 @@ struct BLReplace {
 };
 
 template <typename T, int DIM, int NR, bool FINAL>
-__global__ __launch_bounds__(PT) void k_bl_replace(BLReplace<T> a) {
-  __shared__ double s_red[2 * NR * PT / 64];
+__global__ __launch_bounds__(64 * NR) void k_bl_replace(BLReplace<T> a) {
+  __shared__ double s_red[2 * NR * NR];
   const BLBlock B = a.m.blocks[blockIdx.x];
   const BLInst I = a.m.inst[B.inst];
-  int flag[NR], any = 0;
+  int any = 0;
 #pragma unroll
-  for (int v = 0; v < NR; ++v) {
-    flag[v] = FINAL || a.m.st[int64_t(B.inst) * NR + v].state == 1;
-    any |= flag[v];
-  }
+  for (int k = 0; k < NR; ++k) any |= FINAL || a.m.st[int64_t(B.inst) * NR + k].state == 1;
   if (!any) return;
-  double part[2 * NR];
-#pragma unroll
-  for (int v = 0; v < 2 * NR; ++v) part[v] = 0.0;
-  const int64_t g = B.n0 + threadIdx.x;
-  if (g < B.n1) {
+  const int v = threadIdx.x % NR;
+  const bool flag = FINAL || a.m.st[int64_t(B.inst) * NR + v].state == 1;
+  const int64_t g = B.n0 + threadIdx.x / NR;
+  double rz = 0.0, rr = 0.0;
+  if (g < B.n1 && flag) {
     const int64_t loc = g - I.node_off;
-    int s0 = 0, s1 = 0;
-    const bool own = loc < I.n_owned;
-    if (!own) {
+    const int64_t o = (g * NR + v) * DIM;
+    double kx[DIM];
+    if (loc < I.n_owned) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) kx[c] = a.y64[o + c];
+    } else {
       const int4 fi = a.m.iface[I.if0 + (loc - I.n_owned)];
-      s0 = fi.y;
-      s1 = fi.z;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) kx[c] = 0.0;
+      for (int j = fi.y; j < fi.z; ++j)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) kx[c] += a.slots64[(int64_t(j) * NR + v) * DIM + c];
     }
 #pragma unroll
-    for (int v = 0; v < NR; ++v) {
-      if (!flag[v]) continue;
-      const int64_t o = (g * NR + v) * DIM;
-      double kx[DIM];
-      if (own) {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) kx[c] = a.y64[o + c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) kx[c] = 0.0;
-        for (int j = s0; j < s1; ++j)
-#pragma unroll
-          for (int c = 0; c < DIM; ++c) kx[c] += a.slots64[(int64_t(j) * NR + v) * DIM + c];
-      }
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        const double r64 = a.f[o + c] - kx[c];
-        part[NR + v] += r64 * r64;
-        if constexpr (!FINAL) {
-          const T rv = T(r64);
-          a.r[o + c] = rv;
-          const T zv = a.invD[g * DIM + c] * rv;
-          a.z[o + c] = zv;
-          part[v] += double(rv) * double(zv);
-        }
+    for (int c = 0; c < DIM; ++c) {
+      const double r64 = a.f[o + c] - kx[c];
+      rr += r64 * r64;
+      if constexpr (!FINAL) {
+        const T rv = T(r64);
+        a.r[o + c] = rv;
+        const T zv = a.invD[g * DIM + c] * rv;
+        a.z[o + c] = zv;
+        rz += double(rv) * double(zv);
       }
     }
   }
-  block_sums<2 * NR>(part, s_red);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int v = 0; v < NR; ++v)
-      if (flag[v]) {
-        a.prz[int64_t(blockIdx.x) * NR + v] = part[v];
-        a.prr[int64_t(blockIdx.x) * NR + v] = part[NR + v];
-      }
+  rhs_sums<NR>(rz, rr, s_red);
+  if (threadIdx.x < NR && (FINAL || a.m.st[int64_t(B.inst) * NR + threadIdx.x].state == 1)) {
+    a.prz[int64_t(blockIdx.x) * NR + threadIdx.x] = rz;
+    a.prr[int64_t(blockIdx.x) * NR + threadIdx.x] = rr;
+  }
 }
 
 // After a replacement, between replays (no other kernel runs): for every
@@ -689,7 +690,7 @@ void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p
   u.prr = pnode + size_t(2 * parity + 1) * m.nblk * nr;
   u.parity = parity;
 #define FEMGPU_BL_UPD(D, R) \
-  hipLaunchKernelGGL((k_bl_update<T, D, R>), dim3(m.nblk), dim3(PT), 0, s, u)
+  hipLaunchKernelGGL((k_bl_update<T, D, R>), dim3(m.nblk), dim3(64 * R), 0, s, u)
   if (dim == 2) {
     if (nr == 1) FEMGPU_BL_UPD(2, 1);
     else if (nr == 2) FEMGPU_BL_UPD(2, 2);
@@ -705,7 +706,7 @@ void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p
 template <typename T, bool FINAL>
 static void replace_dispatch(int dim, int nr, const BLReplace<T> &a, hipStream_t s) {
 #define FEMGPU_BL_REP(D, R) \
-  hipLaunchKernelGGL((k_bl_replace<T, D, R, FINAL>), dim3(a.m.nblk), dim3(PT), 0, s, a)
+  hipLaunchKernelGGL((k_bl_replace<T, D, R, FINAL>), dim3(a.m.nblk), dim3(64 * R), 0, s, a)
   if (dim == 2) {
     if (nr == 1) FEMGPU_BL_REP(2, 1);
     else if (nr == 2) FEMGPU_BL_REP(2, 2);
@@ -768,7 +769,6 @@ void bl_final(int dim, int nr, const BLArgs<double> &m64, const double *f, const
   else
     hipLaunchKernelGGL((k_bl_final_sum<4>), dim3(n_inst), dim3(64), 0, s, n_inst, m64.inst, a.prr, m64.st);
 }
-
 
 template void bl_inv_diag<float>(int64_t, const double *, float *, hipStream_t);
 template void bl_inv_diag<double>(int64_t, const double *, double *, hipStream_t);
