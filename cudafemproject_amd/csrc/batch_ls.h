@@ -22,7 +22,7 @@
 namespace femgpu {
 
 constexpr int BL_THREADS = 256;    // patch workgroup (= patch node limit)
-constexpr int BL_UPD_NODES = 64;   // nodes per node-parallel block (64 x NR threads)
+constexpr int BL_UPD_NODES = 256;  // nodes per node-parallel block (BL_UPD_NODES x NR threads)
 
 struct BLInst {
   int32_t node_off, n_owned, n_active, n_nodes;

@@ -342,9 +342,9 @@ __global__ __launch_bounds__(PT) void k_bl_patch(BLPatch<T> a) {
 // BL_UPD_NODES nodes, so that a wave reads contiguous [node][rhs][dim] data.
 
 // Per-rhs block totals of a (thread rhs = lane % NR): xor butterfly over the
-// lanes of one rhs, then the waves in order.  sh: 2 * NR * NR doubles.
-// Totals valid in threads < NR (thread v: rhs v).
-template <int NR>
+// lanes of one rhs, then the W waves of the block in order.  sh: 2 * NR * W
+// doubles.  Totals valid in threads < NR (thread v: rhs v).
+template <int NR, int W>
 __device__ __forceinline__ void rhs_sums(double &a, double &b, double *sh) {
 #pragma unroll
   for (int o = 32; o >= NR; o >>= 1) {
@@ -359,7 +359,7 @@ __device__ __forceinline__ void rhs_sums(double &a, double &b, double *sh) {
   __syncthreads();
   if (threadIdx.x < NR) {
     double sa = 0.0, sb = 0.0;
-    for (int i = 0; i < NR; ++i) {  // NR waves per block
+    for (int i = 0; i < W; ++i) {
       sa += sh[(i * NR + threadIdx.x) * 2];
       sb += sh[(i * NR + threadIdx.x) * 2 + 1];
     }
@@ -381,8 +381,9 @@ struct BLUpdate {
 };
 
 template <typename T, int DIM, int NR>
-__global__ __launch_bounds__(64 * NR) void k_bl_update(BLUpdate<T> a) {
-  __shared__ double s_red[2 * NR * NR];
+__global__ __launch_bounds__(BL_UPD_NODES * NR) void k_bl_update(BLUpdate<T> a) {
+  constexpr int W = BL_UPD_NODES * NR / 64;
+  __shared__ double s_red[2 * NR * W];
   __shared__ double s_al[NR], s_bt[NR];
   __shared__ int s_act[NR];
   const BLBlock B = a.m.blocks[blockIdx.x];
@@ -413,6 +414,10 @@ __global__ __launch_bounds__(64 * NR) void k_bl_update(BLUpdate<T> a) {
     }
   }
   __syncthreads();
+  int any = 0;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) any |= s_act[k];
+  if (!any) return;  // the partials of inactive systems are never read
   const int v = threadIdx.x % NR;
   const int64_t g = B.n0 + threadIdx.x / NR;
   double rz = 0.0, rr = 0.0;
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(64 * NR) void k_bl_update(BLUpdate<T> a) {
       rr += double(rv) * double(rv);
     }
   }
-  rhs_sums<NR>(rz, rr, s_red);
+  rhs_sums<NR, W>(rz, rr, s_red);
   if (threadIdx.x < NR) {
     a.prz[int64_t(blockIdx.x) * NR + threadIdx.x] = rz;
     a.prr[int64_t(blockIdx.x) * NR + threadIdx.x] = rr;
@@ -474,8 +479,9 @@ struct BLReplace {
 };
 
 template <typename T, int DIM, int NR, bool FINAL>
-__global__ __launch_bounds__(64 * NR) void k_bl_replace(BLReplace<T> a) {
-  __shared__ double s_red[2 * NR * NR];
+__global__ __launch_bounds__(BL_UPD_NODES * NR) void k_bl_replace(BLReplace<T> a) {
+  constexpr int W = BL_UPD_NODES * NR / 64;
+  __shared__ double s_red[2 * NR * W];
   const BLBlock B = a.m.blocks[blockIdx.x];
   const BLInst I = a.m.inst[B.inst];
   int any = 0;
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(64 * NR) void k_bl_replace(BLReplace<T> a) {
       }
     }
   }
-  rhs_sums<NR>(rz, rr, s_red);
+  rhs_sums<NR, W>(rz, rr, s_red);
   if (threadIdx.x < NR && (FINAL || a.m.st[int64_t(B.inst) * NR + threadIdx.x].state == 1)) {
     a.prz[int64_t(blockIdx.x) * NR + threadIdx.x] = rz;
     a.prr[int64_t(blockIdx.x) * NR + threadIdx.x] = rr;
@@ -690,7 +696,7 @@ void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p
   u.prr = pnode + size_t(2 * parity + 1) * m.nblk * nr;
   u.parity = parity;
 #define FEMGPU_BL_UPD(D, R) \
-  hipLaunchKernelGGL((k_bl_update<T, D, R>), dim3(m.nblk), dim3(64 * R), 0, s, u)
+  hipLaunchKernelGGL((k_bl_update<T, D, R>), dim3(m.nblk), dim3(BL_UPD_NODES * R), 0, s, u)
   if (dim == 2) {
     if (nr == 1) FEMGPU_BL_UPD(2, 1);
     else if (nr == 2) FEMGPU_BL_UPD(2, 2);
@@ -706,7 +712,7 @@ void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p
 template <typename T, bool FINAL>
 static void replace_dispatch(int dim, int nr, const BLReplace<T> &a, hipStream_t s) {
 #define FEMGPU_BL_REP(D, R) \
-  hipLaunchKernelGGL((k_bl_replace<T, D, R, FINAL>), dim3(a.m.nblk), dim3(64 * R), 0, s, a)
+  hipLaunchKernelGGL((k_bl_replace<T, D, R, FINAL>), dim3(a.m.nblk), dim3(BL_UPD_NODES * R), 0, s, a)
   if (dim == 2) {
     if (nr == 1) FEMGPU_BL_REP(2, 1);
     else if (nr == 2) FEMGPU_BL_REP(2, 2);
