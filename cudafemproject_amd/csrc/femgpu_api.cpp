@@ -1606,10 +1606,11 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     invM[i] = ml_i[i] > 0 ? T(1.0 / ml_i[i]) : T(0);
   }
   const std::string tg = sizeof(T) == 8 ? "d64" : "d32";
-  T *dd, *dv, *da, *dy, *dpart, *dff, *dfu, *dinvM, *dml;
+  T *dd, *dd2, *dv, *da, *dy, *dpart, *dff, *dfu, *dinvM, *dml;
   double *dcf, *dcu, *dmaxp, *dmax;
   int *dstep;
-  if ((st = dbuf(ctx, tg + "d", nd, &dd)) || (st = dbuf(ctx, tg + "v", nd, &dv)) ||
+  if ((st = dbuf(ctx, tg + "d", nd, &dd)) || (st = dbuf(ctx, tg + "d2", nd, &dd2)) ||
+      (st = dbuf(ctx, tg + "v", nd, &dv)) ||
       (st = dbuf(ctx, tg + "a", nd, &da)) || (st = dbuf(ctx, tg + "y", nd, &dy)) ||
       (st = dbuf(ctx, tg + "part", size_t(ctx->plan.n_slots) * d + 1, &dpart)) ||
       (st = upload(ctx, tg + "ff", ff_t, &dff)) || (st = upload(ctx, tg + "fu", fu_t, &dfu)) ||
@@ -1637,9 +1638,11 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   }
   const T *rsrc = ctx->recv_quantity == FEMGPU_RECV_VELOCITY ? dv
                   : ctx->recv_quantity == FEMGPU_RECV_ACCELERATION ? da : dd;
-  auto record = [&](int step_off) {
+  // dsrc: the step's displacement buffer (the explicit steps alternate two)
+  auto record = [&](int step_off, const T *dsrc = nullptr) {
+    const T *src = dsrc && ctx->recv_quantity == FEMGPU_RECV_DISPLACEMENT ? dsrc : rsrc;
     if (nrec && max_samples > 0)
-      femgpu::launch_record<T>(dstep, step_off, every, rsrc, drecv, nrec, d, max_samples, dtr,
+      femgpu::launch_record<T>(dstep, step_off, every, src, drecv, nrec, d, max_samples, dtr,
                                ctx->stream);
   };
   // max |v| for relaxation (device reduction, polled)
@@ -1650,6 +1653,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     return FEMGPU_OK;
   };
   femgpu::launch_fill<T>(dd, int64_t(nd), T(0), ctx->stream);
+  femgpu::launch_fill<T>(dd2, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(dv, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(da, int64_t(nd), T(0), ctx->stream);
   femgpu::launch_fill<T>(dy, int64_t(nd), T(0), ctx->stream);
@@ -1674,10 +1678,15 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   femgpu_dyn_stats S{};
   int steps = 0;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  // explicit steps: the displacement alternates between two buffers (the
+  // fused predictor reads one and writes the other); step n writes dbufs[n & 1 ^ 1]
+  T *dbufs[2] = {dd, dd2};
   if (expl) {
-    // Steps are captured in a HIP graph of G steps when at least one full
-    // graph fits the run; the remainder (or the whole run) is launched eagerly.
-    const int G = o.steps_per_graph > 0 ? o.steps_per_graph : 64;
+    // Steps are captured in a HIP graph of G steps (G even, so every replay
+    // starts on buffer 0) when at least one full graph fits the run; the
+    // remainder (or the whole run) is launched eagerly.
+    int G = o.steps_per_graph > 0 ? o.steps_per_graph : 64;
+    G += G & 1;
     int done = 0;
     bool settled = false;
     if (G <= endnt) {
@@ -1685,11 +1694,11 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < G; ++k) {
-        femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
-        femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+        femgpu::launch_dyn_predict_matvec<T>(ctx->dm, dbufs[k & 1], dv, da, dbufs[(k & 1) ^ 1], C,
+                                             dy, dpart, ctx->stream);
         femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM,
-                                      da, dv, nullptr, C, ctx->stream);
-        record(k);
+                                      da, dv, nullptr, C, ctx->stream, nullptr, true);
+        record(k, dbufs[(k & 1) ^ 1]);
       }
       HIPCHK(hipStreamEndCapture(ctx->stream, &g));
       const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
@@ -1719,11 +1728,12 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     }
     femgpu::launch_set_int(dstep, done + 1, ctx->stream);
     for (int k = 0; done < endnt && !settled; ++k, ++done) {
-      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, false, ctx->stream);
-      femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
+      const int par = done & 1;
+      femgpu::launch_dyn_predict_matvec<T>(ctx->dm, dbufs[par], dv, da, dbufs[par ^ 1], C, dy,
+                                           dpart, ctx->stream);
       femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM, da,
-                                    dv, nullptr, C, ctx->stream);
-      record(k);
+                                    dv, nullptr, C, ctx->stream, nullptr, true);
+      record(k, dbufs[par ^ 1]);
       if (relax) {
         double vm = 0.0;
         if ((st = vmax(&vm))) return st;
@@ -1810,7 +1820,8 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   S.steps = steps;
   if (u_out) {
     std::vector<T> xt(nd);
-    HIPCHK(hipMemcpyAsync(xt.data(), dd, nd * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    const T *dfin = expl ? dbufs[steps & 1] : dd;
+    HIPCHK(hipMemcpyAsync(xt.data(), dfin, nd * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<double> xi(xt.begin(), xt.end());
     from_internal(ctx, xi, u_out);

@@ -140,12 +140,19 @@ struct DynCoeffs {
 template <typename T>
 void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
                         hipStream_t s);
-// r = F(step) - K d (interface finished from partials); explicit: a = r*invM, v += c3 a.
+// r = F(step) - K d (interface finished from partials); explicit: a = r*invM, v += c3 a
+// (pred_v: v and a are the previous step's, the predicted v + c2 a is formed here).
 template <typename T>
 void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T *f_file,
                         const T *f_unit, const double *coef_file, const double *coef_unit,
                         const int *step_base, int step_off, const T *invM, T *a, T *v,
-                        T *r_out, DynCoeffs c, hipStream_t s, const T *mlump = nullptr);
+                        T *r_out, DynCoeffs c, hipStream_t s, const T *mlump = nullptr,
+                        bool pred_v = false);
+// Explicit GN22 predictor fused with the matvec: d_out = (d_in + dt v) + c1 a
+// (owned and first-slot nodes), y / partial = K d_out.
+template <typename T>
+void launch_dyn_predict_matvec(const DevMesh &dm, const T *d_in, const T *v, const T *acc,
+                               T *d_out, DynCoeffs c, T *y, T *partial, hipStream_t s);
 template <typename T>
 void launch_dyn_implicit_post(const DevMesh &dm, const T *a, T *v, T *d, DynCoeffs c,
                               hipStream_t s);

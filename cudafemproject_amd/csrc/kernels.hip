@@ -43,7 +43,8 @@ using namespace patch;
 // ---------------------------------------------------------------------------
 // Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step (stop test and beta
 // from the previous update's partial sums, p = z + beta p, q = cK K p + cM m p,
-// per-patch p'q partial).  MODE 2: diag(K).
+// per-patch p'q partial).  MODE 2: diag(K).  MODE 3: explicit GN22 step, the
+// predictor d~ = d + dt v + c1 a fused with y = K d~.
 
 template <typename T>
 struct PatchArgs {
@@ -70,6 +71,9 @@ struct PatchArgs {
   const double *prz;  // MODE 1: r'z partials of the previous update (NODE_PARTS)
   const double *prr;  // MODE 1: r'r partials of the previous update
   int parity;         // MODE 1: j & 1
+  const T *vel, *acc; // MODE 3: velocity and acceleration of the previous step
+  T *xout;            // MODE 3: predicted displacement (owned and first-slot nodes)
+  T dt, c1;           // MODE 3: GN22 predictor coefficients
 };
 
 // One workgroup per patch, one patch node per thread (phases A and C), up to
@@ -183,6 +187,13 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
       }
+    } else if constexpr (MODE == 3) {
+      // the order of k_dyn_predict: (d + dt v) + c1 a
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const T dv = a.x[gid * DIM + c] + a.dt * a.vel[gid * DIM + c];
+        xin[c] = dv + a.c1 * a.acc[gid * DIM + c];
+      }
     }
     if constexpr (MODE != 2) {
 #pragma unroll
@@ -205,6 +216,8 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
     }
   } else if constexpr (MODE == 0) {
     if (has_node && !own && fm) first = slot == a.slot_start[gid - a.n_owned];
+  } else if constexpr (MODE == 3) {
+    if (has_node && !own) first = slot == a.slot_start[gid - a.n_owned];
   }
   __syncthreads();
 
@@ -241,6 +254,11 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
     } else {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = MODE == 1 ? cK * acc[c] : acc[c];
+    }
+    if constexpr (MODE == 3) {
+      if (first)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) a.xout[gid * DIM + c] = xin[c];
     }
   }
 
@@ -721,7 +739,7 @@ __global__ __launch_bounds__(NT) void k_dyn_correct(int64_t n_owned, int64_t n_a
                                                     const double *coef_unit, const int *step_base,
                                                     int step_off, const T *invM, T *a, T *v,
                                                     T *r_out, T c3, const T *mlump, T alpha,
-                                                    T scale) {
+                                                    T scale, int pred_v, T c2) {
   const int step = *step_base + step_off;
   const T cf = T(coef_file[step]), cu = T(coef_unit[step]);
   const int64_t stride = int64_t(gridDim.x) * NT;
@@ -742,10 +760,13 @@ __global__ __launch_bounds__(NT) void k_dyn_correct(int64_t n_owned, int64_t n_a
       const T r = cf * f_file[o] + cu * f_unit[o] - kd[c];
       if (invM) {
         // explicit; with mass-proportional damping alpha M:
-        // (1 + alpha c3) M a = F - K d - alpha M v_pred
-        const T av = alpha != T(0) ? (r * invM[o] - alpha * v[o]) * scale : r * invM[o];
+        // (1 + alpha c3) M a = F - K d - alpha M v_pred.  pred_v: v holds the
+        // previous step's velocity and a its acceleration, v_pred = v + c2 a
+        // is formed here (the predictor of k_dyn_predict, fused away)
+        const T vp = pred_v ? v[o] + c2 * a[o] : v[o];
+        const T av = alpha != T(0) ? (r * invM[o] - alpha * vp) * scale : r * invM[o];
         a[o] = av;
-        v[o] = v[o] + c3 * av;
+        v[o] = vp + c3 * av;
       } else {
         r_out[o] = alpha != T(0) ? r - alpha * mlump[o] * v[o] : r;
       }
@@ -892,10 +913,11 @@ hipError_t prepare_kernels() {
   hipError_t e = hipSuccess, r;
 #define FEMGPU_ATTR(T, D, M, P) \
   if ((r = patch_attr<T, D, M, P>()) != hipSuccess) e = r;
-#define FEMGPU_ATTR_TD(T, D)                                                   \
-  FEMGPU_ATTR(T, D, 0, 256) FEMGPU_ATTR(T, D, 1, 256) FEMGPU_ATTR(T, D, 2, 256)     \
-  FEMGPU_ATTR(T, D, 0, 512) FEMGPU_ATTR(T, D, 1, 512) FEMGPU_ATTR(T, D, 2, 512)     \
-  FEMGPU_ATTR(T, D, 0, 1024) FEMGPU_ATTR(T, D, 1, 1024) FEMGPU_ATTR(T, D, 2, 1024)
+#define FEMGPU_ATTR_TD(T, D)                                                              \
+  FEMGPU_ATTR(T, D, 0, 256) FEMGPU_ATTR(T, D, 1, 256) FEMGPU_ATTR(T, D, 2, 256)                \
+  FEMGPU_ATTR(T, D, 3, 256) FEMGPU_ATTR(T, D, 0, 512) FEMGPU_ATTR(T, D, 1, 512)                \
+  FEMGPU_ATTR(T, D, 2, 512) FEMGPU_ATTR(T, D, 3, 512) FEMGPU_ATTR(T, D, 0, 1024)               \
+  FEMGPU_ATTR(T, D, 1, 1024) FEMGPU_ATTR(T, D, 2, 1024) FEMGPU_ATTR(T, D, 3, 1024)
   FEMGPU_ATTR_TD(double, 3)
   FEMGPU_ATTR_TD(double, 2)
   FEMGPU_ATTR_TD(float, 3)
@@ -1121,16 +1143,36 @@ template <typename T>
 void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T *f_file,
                         const T *f_unit, const double *coef_file, const double *coef_unit,
                         const int *step_base, int step_off, const T *invM, T *a, T *v, T *r_out,
-                        DynCoeffs c, hipStream_t s, const T *mlump) {
+                        DynCoeffs c, hipStream_t s, const T *mlump, bool pred_v) {
   const int g = grid_for(dm.n_active);
   if (dm.dim == 3)
     hipLaunchKernelGGL((k_dyn_correct<T, 3>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
                        dm.slot_start, y, partial, f_file, f_unit, coef_file, coef_unit, step_base,
-                       step_off, invM, a, v, r_out, T(c.c3), mlump, T(c.alpha), T(c.expl_scale));
+                       step_off, invM, a, v, r_out, T(c.c3), mlump, T(c.alpha), T(c.expl_scale),
+                       pred_v ? 1 : 0, T(c.c2));
   else
     hipLaunchKernelGGL((k_dyn_correct<T, 2>), dim3(g), dim3(NT), 0, s, dm.n_owned, dm.n_active,
                        dm.slot_start, y, partial, f_file, f_unit, coef_file, coef_unit, step_base,
-                       step_off, invM, a, v, r_out, T(c.c3), mlump, T(c.alpha), T(c.expl_scale));
+                       step_off, invM, a, v, r_out, T(c.c3), mlump, T(c.alpha), T(c.expl_scale),
+                       pred_v ? 1 : 0, T(c.c2));
+}
+
+template <typename T>
+void launch_dyn_predict_matvec(const DevMesh &dm, const T *d_in, const T *v, const T *acc,
+                               T *d_out, DynCoeffs c, T *y, T *partial, hipStream_t s) {
+  PatchArgs<T> a = patch_args<T>(dm);
+  a.x = d_in;
+  a.vel = v;
+  a.acc = acc;
+  a.xout = d_out;
+  a.dt = T(c.dt);
+  a.c1 = T(c.c1);
+  a.y = y;
+  a.partial = partial;
+  if (dm.dim == 3)
+    launch_patch<T, 3, 3>(dm, a, s);
+  else
+    launch_patch<T, 2, 3>(dm, a, s);
 }
 
 template <typename T>
@@ -1189,7 +1231,9 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_dyn_correct<T>(const DevMesh &, const T *, const T *, const T *,        \
                                       const T *, const double *, const double *, const int *,  \
                                       int, const T *, T *, T *, T *, DynCoeffs, hipStream_t,   \
-                                      const T *);                                              \
+                                      const T *, bool);                                        \
+  template void launch_dyn_predict_matvec<T>(const DevMesh &, const T *, const T *, const T *, \
+                                             T *, DynCoeffs, T *, T *, hipStream_t);          \
   template void launch_dyn_implicit_post<T>(const DevMesh &, const T *, T *, T *, DynCoeffs,   \
                                             hipStream_t);                                      \
   template void launch_xpby<T>(int64_t, const T *, const T *, double, T *, hipStream_t);       \
