@@ -37,6 +37,107 @@ uint64_t spread2(uint64_t v) {  // 31 bits -> every second bit
   return v;
 }
 
+// Entry positions within the padded node runs of one patch.  The patch
+// kernels write the entries with one instruction per (element slot e,
+// corner a): thread t writes the entry of its element t + e * pn_max.  Any
+// position of a node's run may hold any of the node's entries (the padding
+// positions are the unused ones, which stay zero), so the positions are
+// chosen to spread the lanes of each write instruction over the LDS banks
+// (4-byte words: 64 banks per wave, 32 per half wave): a greedy pass in
+// the kernel's write order, then two sweeps of pairwise swaps inside each run
+// that lower sum_W (2 * sum_b n64(W,b)^2 + sum_h,b n32(W,h,b)^2), n = lanes of
+// write instruction W on a bank.  The order of the entries inside a run is
+// the summation order of the node's sum; it is fixed by the plan.
+struct BankScratch {
+  std::vector<int32_t> occ, rstart, wid;
+  std::vector<uint8_t> half;
+  std::vector<int16_t> c64, c32;
+};
+
+std::string spread_entry_banks(MeshPlan &plan, const PatchInfo &P, int npe, int pn_max,
+                               int32_t nloc, int32_t run, BankScratch &B) {
+  const int32_t nwave = (pn_max + 63) / 64;
+  const int32_t slots_e = (P.n_elem + pn_max - 1) / pn_max;
+  const int32_t n_w = slots_e * npe * nwave;
+  B.occ.assign(size_t(run), -1);
+  B.rstart.resize(nloc);
+  for (int32_t k = 0; k < nloc; ++k) B.rstart[k] = k ? int32_t(plan.node_end[P.pnode_start + k - 1]) : 0;
+  B.wid.assign(size_t(P.n_elem) * 4, 0);
+  B.half.assign(size_t(P.n_elem) * 4, 0);
+  B.c64.assign(size_t(n_w) * 64, 0);
+  B.c32.assign(size_t(n_w) * 64, 0);
+  uint16_t *epos = &plan.epos[size_t(P.elem_start) * 4];
+  const uint16_t *lid = &plan.elem_lid[size_t(P.elem_start) * 4];
+  for (int32_t e = 0; e < slots_e; ++e)
+    for (int a = 0; a < npe; ++a)
+      for (int32_t t = 0; t < pn_max; ++t) {
+        const int32_t i = e * pn_max + t;
+        if (i >= P.n_elem) break;
+        const int32_t j = i * 4 + a, l = lid[j];
+        const int32_t W = (e * npe + a) * nwave + t / 64, h = (t & 63) >> 5;
+        int16_t *u64 = &B.c64[size_t(W) * 64], *u32 = &B.c32[size_t(W) * 64 + h * 32];
+        const int32_t r1 = int32_t(plan.node_end[P.pnode_start + l]);
+        int32_t best = -1, best_cost = INT32_MAX;
+        for (int32_t p = B.rstart[l]; p < r1; ++p) {
+          if (B.occ[p] >= 0) continue;
+          const int32_t cost = 2 * u64[p & 63] + u32[p & 31];
+          if (cost < best_cost) {
+            best_cost = cost;
+            best = p;
+            if (cost == 0) break;
+          }
+        }
+        if (best < 0) return "internal error: node run full";
+        B.occ[best] = j;
+        B.wid[j] = W;
+        B.half[j] = uint8_t(h);
+        ++u64[best & 63];
+        ++u32[best & 31];
+      }
+  // cost change of moving entry j from position p to q (applied)
+  auto move = [&](int32_t j, int32_t p, int32_t q) {
+    int16_t *u64 = &B.c64[size_t(B.wid[j]) * 64], *u32 = &B.c32[size_t(B.wid[j]) * 64 + B.half[j] * 32];
+    int32_t d = 0;
+    if ((p ^ q) & 63) {
+      d += 4 * (u64[q & 63] - u64[p & 63] + 1);
+      --u64[p & 63];
+      ++u64[q & 63];
+    }
+    if ((p ^ q) & 31) {
+      d += 2 * (u32[q & 31] - u32[p & 31] + 1);
+      --u32[p & 31];
+      ++u32[q & 31];
+    }
+    return d;
+  };
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    int32_t improved = 0;
+    for (int32_t k = 0; k < nloc; ++k) {
+      const int32_t r0 = B.rstart[k], r1 = int32_t(plan.node_end[P.pnode_start + k]);
+      for (int32_t p = r0; p < r1; ++p)
+        for (int32_t q = p + 1; q < r1; ++q) {
+          const int32_t x = B.occ[p], y = B.occ[q];
+          if (x < 0 && y < 0) continue;
+          int32_t d = 0;
+          if (x >= 0) d += move(x, p, q);
+          if (y >= 0) d += move(y, q, p);
+          if (d < 0) {
+            B.occ[p] = y;
+            B.occ[q] = x;
+            ++improved;
+          } else {
+            if (y >= 0) move(y, p, q);
+            if (x >= 0) move(x, q, p);
+          }
+        }
+    }
+    if (!improved) break;
+  }
+  for (int32_t p = 0; p < run; ++p)
+    if (B.occ[p] >= 0) epos[B.occ[p]] = uint16_t(p);
+  return "";
+}
+
 }  // namespace
 
 std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
@@ -180,6 +281,7 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
   std::vector<int32_t> local(nn, -1);
   std::vector<int32_t> loc_nodes;
   std::vector<int32_t> cnt;
+  BankScratch bank;
   for (int32_t p = 0; p < np; ++p) {
     PatchInfo &P = plan.patches[p];
     P.elem_start = pstart[p];
@@ -246,11 +348,16 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
     }
     if (run >= 16384) return "internal error: entry list above 14-bit positions";
     plan.ent_used = std::max(plan.ent_used, int(run));
-    for (int32_t i = 0; i < P.n_elem; ++i)
-      for (int a = 0; a < npe; ++a) {
-        int32_t l = plan.elem_lid[size_t(P.elem_start + i) * 4 + a];
-        plan.epos[size_t(P.elem_start + i) * 4 + a] = uint16_t(pos[l]++);
-      }
+    if (!pad_runs) {
+      for (int32_t i = 0; i < P.n_elem; ++i)
+        for (int a = 0; a < npe; ++a) {
+          int32_t l = plan.elem_lid[size_t(P.elem_start + i) * 4 + a];
+          plan.epos[size_t(P.elem_start + i) * 4 + a] = uint16_t(pos[l]++);
+        }
+    } else {
+      const std::string err = spread_entry_banks(plan, P, npe, pn_max, nloc, run, bank);
+      if (!err.empty()) return err;
+    }
     for (int32_t v : loc_nodes) local[v] = -1;
   }
   return "";
