@@ -76,6 +76,45 @@ def cpu_baseline(pb, n_fp64_iters, cpu_iters):
     }
 
 
+def cpu_efficient_baseline(pb, n_fp64_iters, cpu_iters):
+    """The "efficient CPU" line: the FP64 restatement with its loops spread
+    over OpenMP threads (one per host core share), setup + a bounded number
+    of PCG iterations, extrapolated like cpu_baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
+    setup_s, iter_s = O.static_time_omp_f64(pb, cpu_iters, threads)
+    est = setup_s + iter_s / cpu_iters * n_fp64_iters
+    return {
+        "value": est, "unit": "s", "cores": threads, "kind": "port-openmp-fp64",
+        "sample": (f"cube64: FP64 restatement, {threads} OpenMP threads: setup {setup_s:.2f} s + "
+                   f"{cpu_iters} PCG iterations {iter_s:.2f} s, extrapolated to "
+                   f"{n_fp64_iters} iterations"),
+    }
+
+
+def copy_gbs(reps=20):
+    """Device-to-device copy bandwidth (read + write bytes / time) of a 256 MiB
+    float32 tensor, event-timed: the measured-copy ceiling.  Runs in a child
+    process: torch ships its own HIP runtime, which cannot open the GPU in a
+    process where the library's runtime already did."""
+    code = (
+        "import torch\n"
+        "x = torch.empty(64 << 20, dtype=torch.float32, device='cuda'); y = torch.empty_like(x)\n"
+        "x.fill_(1.0)\n"
+        "[y.copy_(x) for _ in range(3)]\n"
+        "e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)\n"
+        f"e0.record(); [y.copy_(x) for _ in range({reps})]; e1.record()\n"
+        "torch.cuda.synchronize()\n"
+        f"print(2 * (256 << 20) / (e0.elapsed_time(e1) / 1e3 / {reps}) / 1e9)\n")
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        log(f"device copy timing failed: {r.stderr.strip()[-300:]}")
+        return None
+    return float(r.stdout.strip().splitlines()[-1])
+
+
 def lumped_mass(pb, rho):
     """Physical lumped mass per node: rho V / 4 of every tet at each corner."""
     X = pb.nodes[pb.elems]
@@ -346,6 +385,10 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="hexes per cube edge")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-copy-probe", action="store_true",
+                    help="skip the device-copy bandwidth child process")
+    ap.add_argument("--no-single-reduction", action="store_true",
+                    help="cube: skip the single-reduction recurrence comparison")
     ap.add_argument("--matvec-reps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -399,15 +442,19 @@ def main():
         f"rel_res {smx['rel_res']:.2e}, |u_mixed-u_fp64|/|u_fp64| {err:.2e}")
     # the same solves with the single-reduction recurrence (one launch per
     # iteration); the FP64 baseline of the speedup is the faster of the two
-    run(F.FP64, 1, F.CG_SINGLE_REDUCTION)
-    run(F.MIXED, 1, F.CG_SINGLE_REDUCTION)
-    r64s = run(F.FP64, args.steps, F.CG_SINGLE_REDUCTION)
-    rmxs = run(F.MIXED, args.steps, F.CG_SINGLE_REDUCTION)
-    t64s = np.mean([s["solve_ms"] for _, s in r64s]) / 1e3
-    tmxs = np.mean([s["solve_ms"] for _, s in rmxs]) / 1e3
-    log(f"single-reduction recurrence: fp64 {t64s * 1e3:.2f} ms ({r64s[-1][1]['iterations']} it), "
-        f"mixed {tmxs * 1e3:.2f} ms ({rmxs[-1][1]['iterations']} it)")
-    t64_best = min(t64, t64s)
+    t64s = tmxs = None
+    it64s = itmxs = None
+    if not args.no_single_reduction:
+        run(F.FP64, 1, F.CG_SINGLE_REDUCTION)
+        run(F.MIXED, 1, F.CG_SINGLE_REDUCTION)
+        r64s = run(F.FP64, args.steps, F.CG_SINGLE_REDUCTION)
+        rmxs = run(F.MIXED, args.steps, F.CG_SINGLE_REDUCTION)
+        t64s = float(np.mean([s["solve_ms"] for _, s in r64s]) / 1e3)
+        tmxs = float(np.mean([s["solve_ms"] for _, s in rmxs]) / 1e3)
+        it64s, itmxs = r64s[-1][1]["iterations"], rmxs[-1][1]["iterations"]
+        log(f"single-reduction recurrence: fp64 {t64s * 1e3:.2f} ms ({it64s} it), "
+            f"mixed {tmxs * 1e3:.2f} ms ({itmxs} it)")
+    t64_best = min(t64, t64s) if t64s else t64
 
     mv32 = ctx.time_matvec(F.FP32, args.matvec_reps) / 1e3
     mv64 = ctx.time_matvec(F.FP64, args.matvec_reps) / 1e3
@@ -423,11 +470,17 @@ def main():
                 "kernel": "EbE matvec fp32 (k_ebe_patch + k_iface_gather), cube64",
                 "bytes_per_launch": b32}
 
-    cpu = None
+    cgbs = None if args.no_copy_probe else copy_gbs()
+    if cgbs:
+        log(f"device copy {cgbs:.0f} GB/s; fp32 matvec at {achieved / cgbs:.3f} of it")
+
+    cpu = cpu_eff = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         cpu = cpu_baseline(pb, s64["iterations"], args.cpu_iters)
         log(f"cpu baseline: {cpu['value']:.1f} s ({cpu['sample']})")
+        cpu_eff = cpu_efficient_baseline(pb, s64["iterations"], args.cpu_iters)
+        log(f"efficient cpu baseline: {cpu_eff['value']:.1f} s ({cpu_eff['sample']})")
 
     line = {
         "metric": "3D PCG time-to-solve @ rel-res 1e-8, mixed vs FP64; EbE matvec HBM GB/s",
@@ -441,10 +494,10 @@ def main():
                    "patch_tuning": {"threads": 256, "elements": 768}},
         "fp64_time_to_solve_s": float(t64),
         "speedup_mixed_vs_fp64": float(t64_best / tmx),
-        "fp64_single_reduction_time_to_solve_s": float(t64s),
-        "mixed_single_reduction_time_to_solve_s": float(tmxs),
-        "iterations_single_reduction_fp64": r64s[-1][1]["iterations"],
-        "iterations_single_reduction_mixed": rmxs[-1][1]["iterations"],
+        "fp64_single_reduction_time_to_solve_s": t64s,
+        "mixed_single_reduction_time_to_solve_s": tmxs,
+        "iterations_single_reduction_fp64": it64s,
+        "iterations_single_reduction_mixed": itmxs,
         "mixed_vs_fp64_rel_displacement_diff": err,
         "iterations_fp64": s64["iterations"], "iterations_mixed": smx["iterations"],
         "outer_iterations_mixed": smx["outer_iterations"],
@@ -454,8 +507,10 @@ def main():
         "matvec_us_fp32": mv32 * 1e6, "matvec_us_fp64": mv64 * 1e6,
         "matvec_gbs_fp32": b32 / mv32 / 1e9, "matvec_gbs_fp64": b64 / mv64 / 1e9,
         "dof_matvecs_per_s_fp32": pb.n_dof / mv32, "dof_matvecs_per_s_fp64": pb.n_dof / mv64,
+        "copy_gbs": cgbs, "matvec_frac_of_copy_fp32": achieved / cgbs if cgbs else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_efficient_baseline": cpu_eff,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

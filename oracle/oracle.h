@@ -111,6 +111,10 @@ int orc_static_solve_f64(const orc_problem *pb, double eps, int max_iter,
  * in setup and in the iterations. */
 int orc_static_time_f32(const orc_problem *pb, int n_iter,
                         double *setup_s, double *iter_s);
+/* The same timing for the FP64 restatement with its loops spread over
+ * n_threads OpenMP threads (n_threads <= 0: OpenMP's default). */
+int orc_static_time_omp_f64(const orc_problem *pb, int n_iter, int n_threads,
+                            double *setup_s, double *iter_s);
 
 /* Dynamics: gpuCalculateFEM_DYN (femfunc.cpp:130-381). u_out: n_nodes x dim. */
 int orc_dyn_solve_f32(const orc_problem *pb, const orc_dyn_params *dp,

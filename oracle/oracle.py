@@ -203,6 +203,18 @@ def static_time_f32(pb, n_iter):
     return a.value, b.value
 
 
+def static_time_omp_f64(pb, n_iter, n_threads):
+    """(setup_s, iter_s) of the FP64 restatement with OpenMP loops."""
+    pin = _Pinned(pb)
+    a, b = C.c_double(0), C.c_double(0)
+    f = lib().orc_static_time_omp_f64
+    f.argtypes = [C.POINTER(OrcProblem), C.c_int, C.c_int, C.POINTER(C.c_double),
+                  C.POINTER(C.c_double)]
+    st = f(C.byref(pin.s), n_iter, n_threads, C.byref(a), C.byref(b))
+    assert st == ORC_OK, st
+    return a.value, b.value
+
+
 def wavelet_value(t, wavelet):
     f = lib().orc_wavelet_value
     f.restype = C.c_float

@@ -280,3 +280,14 @@ def test_scramble_flag_matches_uniform_mass():
     a = O.dyn_solve(pb, 1000.0, 0.5, 0.0, 1e-4, 1e-3, None, np.float64, True)[1]
     b = O.dyn_solve(pb, 1000.0, 0.5, 0.0, 1e-4, 1e-3, None, np.float64, False)[1]
     assert rel(a, b) < 1e-12
+
+
+def test_openmp_timing_baseline_runs():
+    """The OpenMP FP64 timing variant (bench.py's "efficient CPU" line) runs
+    a bounded number of iterations on any thread count; it is a timer only,
+    no result of it is used as a reference."""
+    pb = P.cube_problem(4)
+    for threads in (1, 2):
+        setup_s, iter_s = O.static_time_omp_f64(pb, 3, threads)
+        assert np.isfinite(setup_s) and np.isfinite(iter_s)
+        assert setup_s >= 0.0 and iter_s >= 0.0
