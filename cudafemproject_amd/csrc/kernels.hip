@@ -201,17 +201,16 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
     }
   }
   // interface nodes: does this patch hold the node's first slot?  (The
-  // first patch writes the constrained rows and the node's mass and
-  // constrained-diagonal terms of p'(cK K + cM M) p.)
+  // first patch writes the constrained rows and, in the PCG step, adds the
+  // node's mass term of p'(cK K + cM M) p.)
   bool first = own;
   if constexpr (MODE == 1) {
     if (has_node && !own && (a.sh.m || fm)) first = slot == a.slot_start[gid - a.n_owned];
-    if (has_node && first) {
+    if (has_node && !own && first && a.sh.m) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         const double pv = double(xin[c]);
-        if (a.sh.m) acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * pv * pv;
-        if ((fm >> c) & 1u) acc_en += double(cK) * double(a.diagK[gid * DIM + c]) * pv * pv;
+        acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * pv * pv;
       }
     }
   } else if constexpr (MODE == 0) {
@@ -223,8 +222,11 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
 
   // Phases B and C: element forces, node-sorted LDS assembly, node sums.
   T acc[DIM];
-  assemble<T, DIM, PT, MODE == 2>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0, b1, acc,
-                                  acc_en, double(cK));
+  // The PCG step forms p'q node by node below (sum over the patch's nodes of
+  // p times what the patch writes for the node: its q row or its slot), not
+  // element by element: sum_e p_e'K_e p_e regrouped, with fewer operations.
+  assemble<T, DIM, PT, MODE == 2, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0,
+                                         b1, acc, acc_en, double(cK));
 
   if (has_node) {
     // rows of constrained components: the assembled diagonal only
@@ -244,6 +246,7 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
         if constexpr (MODE == 1) {
           v = cK * v;
           if (a.sh.m) v += cM * a.sh.m[gid * DIM + c] * xin[c];
+          acc_en += double(xin[c]) * double(v);
         }
         a.y[gid * DIM + c] = v;
       }
@@ -253,7 +256,11 @@ __global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = MODE == 1 ? cK * acc[c] : acc[c];
+      for (int c = 0; c < DIM; ++c) {
+        const T v = MODE == 1 ? cK * acc[c] : acc[c];
+        if constexpr (MODE == 1) acc_en += double(xin[c]) * double(v);
+        a.partial[slot * DIM + c] = v;
+      }
     }
     if constexpr (MODE == 3) {
       if (first)

@@ -87,9 +87,10 @@ __device__ __forceinline__ void clear_entries(T *s_res, int ent) {
 // C, one component at a time: entries to their node-sorted LDS positions,
 // then every node sums its contiguous run [b0, b1) in order (runs start and
 // end on multiples of 4 entries: 16-byte reads, two in flight per step).
-// acc receives this thread's node sums; en_acc += en_scale * u_e'K_e u_e for
-// each of this thread's elements (DIAG false; DIAG: the element diagonals).
-template <typename T, int DIM, int PT, bool DIAG>
+// acc receives this thread's node sums; with EN, en_acc += en_scale *
+// u_e'K_e u_e for each of this thread's elements (DIAG false; DIAG: the
+// element diagonals).
+template <typename T, int DIM, int PT, bool DIAG, bool EN = true>
 __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, const uint3 *rec,
                                          const T *esc, int n_elem, T lam, T mu, int b0, int b1,
                                          T *acc, double &en_acc, double en_scale) {
@@ -114,7 +115,7 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
       elem::elem_diag<T, DIM>(X, lam, mu, esc[e], F[e]);
     } else {
       const T en = elem::elem_force_free<T, DIM>(X, U, lam, mu, esc[e], F[e]);
-      en_acc += en_scale * double(en);
+      if constexpr (EN) en_acc += en_scale * double(en);
     }
   }
 #pragma unroll
