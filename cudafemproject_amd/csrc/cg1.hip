@@ -235,7 +235,6 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
       for (int c = 0; c < DIM; ++c) {
         acc_g += double(rn[c]) * double(u[c]);
         acc_q += double(rn[c]) * double(rn[c]);
-        if ((fm >> c) & 1u) acc_d += double(a.diagK[o + c]) * double(u[c]) * double(u[c]);
       }
     }
 #pragma unroll
@@ -243,10 +242,11 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   }
   __syncthreads();
 
-  // Phases B and C: w_{j+1} = A u_{j+1}
+  // Phases B and C: w_{j+1} = A u_{j+1}; d = u'w is summed node by node
+  // below from what the patch writes (its rows of w and its slots)
   T acc[DIM];
-  assemble<T, DIM, PT, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0, b1, acc, acc_d,
-                              1.0);
+  assemble<T, DIM, PT, false, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0, b1, acc,
+                                     acc_d, 1.0);
   if (has_node) {
     // constrained rows: the assembled diagonal only (applyConstraints,
     // cpu_fem_utils.cpp:461-476), written by the node's first patch
@@ -255,6 +255,8 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
       for (int c = 0; c < DIM; ++c)
         if ((fm >> c) & 1u) acc[c] = first ? a.diagK[gid * DIM + c] * u[c] : T(0);
     }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) acc_d += double(u[c]) * double(acc[c]);
     if (own) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) S.w[gid * DIM + c] = acc[c];

@@ -104,13 +104,18 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
     decode_rec(rec[e], l, q[e]);
     if (i >= n_elem) continue;
     T X[NDE], U[NDE];
+    // byte offset of the corner's node row, shared by s_c and s_x
 #pragma unroll
-    for (int n = 0; n < NPE; ++n)
+    for (int n = 0; n < NPE; ++n) {
+      const unsigned bo = unsigned(l[n]) * unsigned(DIM * sizeof(T));
+      const T *xc = reinterpret_cast<const T *>(reinterpret_cast<const char *>(s_c) + bo);
+      const T *xu = reinterpret_cast<const T *>(reinterpret_cast<const char *>(s_x) + bo);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        X[n * DIM + c] = s_c[l[n] * DIM + c];
-        if constexpr (!DIAG) U[n * DIM + c] = s_x[l[n] * DIM + c];
+        X[n * DIM + c] = xc[c];
+        if constexpr (!DIAG) U[n * DIM + c] = xu[c];
       }
+    }
     if constexpr (DIAG) {
       elem::elem_diag<T, DIM>(X, lam, mu, esc[e], F[e]);
     } else {
@@ -129,17 +134,17 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
     using V4 = typename Vec4<T>::type;
     const V4 *r4 = reinterpret_cast<const V4 *>(s_res);
     T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-    int j = b0 >> 2;
-    const int j1 = b1 >> 2;
-    for (; j + 2 <= j1; j += 2) {
-      const V4 u = r4[j], v = r4[j + 1];
+    const V4 *rp = r4 + (b0 >> 2);
+    const int n4 = (b1 - b0) >> 2;
+    for (int m = n4 >> 1; m > 0; --m, rp += 2) {
+      const V4 u = rp[0], v = rp[1];
       s0 += Vec4<T>::get(u, 0) + Vec4<T>::get(v, 0);
       s1 += Vec4<T>::get(u, 1) + Vec4<T>::get(v, 1);
       s2 += Vec4<T>::get(u, 2) + Vec4<T>::get(v, 2);
       s3 += Vec4<T>::get(u, 3) + Vec4<T>::get(v, 3);
     }
-    if (j < j1) {
-      const V4 u = r4[j];
+    if (n4 & 1) {
+      const V4 u = rp[0];
       s0 += Vec4<T>::get(u, 0);
       s1 += Vec4<T>::get(u, 1);
       s2 += Vec4<T>::get(u, 2);
