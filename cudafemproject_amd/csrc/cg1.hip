@@ -136,12 +136,15 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   const int nloc = P.n_own + P.n_sh;
   uint3 rec[EPT];
   T esc[EPT];
+  // per-patch bases (uniform) and small unsigned thread offsets: scalar
+  // base + one offset register per load
+  const uint3 *prec = reinterpret_cast<const uint3 *>(a.erec) + P.elem_start;
+  const T *pesc = a.escale ? a.escale + P.elem_start : nullptr;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * PT;
-    rec[e] = i < P.n_elem ? reinterpret_cast<const uint3 *>(a.erec)[P.elem_start + i]
-                          : make_uint3(0u, 0u, 0u);
-    esc[e] = a.escale && i < P.n_elem ? a.escale[P.elem_start + i] : T(1);
+    rec[e] = i < P.n_elem ? prec[unsigned(i)] : make_uint3(0u, 0u, 0u);
+    esc[e] = pesc && i < P.n_elem ? pesc[unsigned(i)] : T(1);
   }
   clear_entries<T, PT>(s_res, a.ent);
 
