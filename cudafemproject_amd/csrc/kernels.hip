@@ -80,8 +80,19 @@ struct PatchArgs {
 // EPT elements per thread (phase B).  The element forces stay in registers
 // and are assembled one component at a time through a single-component LDS
 // entry list, which keeps the LDS per element at npe words.
+// Waves per SIMD asked of the register allocator.  The FP64 matvec and PCG
+// step of 256-thread patches fit 128 VGPRs with a few spilled words: four
+// resident workgroups per CU instead of three (FP64 matvec 29.1 -> 28.3 us,
+// PCG iteration -0.3 us at cube64).  The FP32 PCG step at six waves spills
+// and is slower; everything else keeps the allocator's choice.
+template <typename T, int MODE, int PT>
+struct PatchWaves {
+  static constexpr int value = PT == 256 && sizeof(T) == 8 && (MODE == 0 || MODE == 1) ? 4 : 1;
+};
+
 template <typename T, int DIM, int MODE, int PT>
-__global__ __launch_bounds__(PT) void k_ebe_patch(PatchArgs<T> a) {
+__global__ __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(PatchWaves<T, MODE, PT>::value)))
+void k_ebe_patch(PatchArgs<T> a) {
   constexpr unsigned FMASK = (1u << DIM) - 1u;
   extern __shared__ __align__(16) unsigned char lds[];
   const int b = blockIdx.x;
