@@ -254,6 +254,10 @@ int solve_lockstep(femgpu_ctx *ctx, BatchState *b, int n_rhs, int nr, const doub
   BL_BUF(t + "s", ns, &sT);
   BL_BUF(t + "invD", nd, &invD);
   BL_BUF(t + "diag", nd, &diagT);
+  // mixed: the replay's solution increments in FP32, added to the FP64
+  // solution before each replacement
+  T *xinc = nullptr;
+  if constexpr (MIXED) BL_BUF(t + "xinc", nv, &xinc);
   auto fill = [&](auto &a) {
     a.patches = static_cast<const PatchInfo *>(b->bufs["bl_patches"].first);
     a.erec = static_cast<const uint32_t *>(b->bufs["bl_erec"].first);
@@ -294,6 +298,7 @@ int solve_lockstep(femgpu_ctx *ctx, BatchState *b, int n_rhs, int nr, const doub
   if constexpr (MIXED) launch_convert<double, T>(diag64, diagT, int64_t(nd), s);
   launch_fill<double>(x64, int64_t(nv), 0.0, s);
   launch_fill<T>(p, int64_t(nv), T(0), s);
+  if (xinc) launch_fill<T>(xinc, int64_t(nv), T(0), s);
   bl_init_state(n_state, st, s);
   bl_replace<T>(dim, nr, m, m64, d_f, x64, y64, s64, invD, r, z, pnode, n_state, d_count, s);
   BCHK_(hipGetLastError());
@@ -302,7 +307,8 @@ int solve_lockstep(femgpu_ctx *ctx, BatchState *b, int n_rhs, int nr, const doub
   hipGraphExec_t exec = nullptr;
   BCHK_(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < REPLAY; ++k)
-    bl_iteration<T>(dim, nr, m, invD, z, p, q, r, sT, x64, ppq, pnode, k & 1, s);
+    bl_iteration<T>(dim, nr, m, invD, z, p, q, r, sT, x64, xinc, ppq, pnode, k & 1, s);
+  if constexpr (MIXED) bl_flush<T>(dim, nr, m, xinc, x64, s);
   bl_replace<T>(dim, nr, m, m64, d_f, x64, y64, s64, invD, r, z, pnode, n_state, d_count, s);
   BCHK_(hipStreamEndCapture(s, &g));
   const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);

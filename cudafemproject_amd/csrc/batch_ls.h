@@ -81,10 +81,15 @@ void bl_inv_diag(int64_t n, const double *diag, T *invD, hipStream_t s);
 // per (instance, rhs): state 1 (replacement requested), counters reset
 void bl_init_state(int n_state, BLState *st, hipStream_t s);
 // One PCG iteration (patch step + node update) at launch parity `parity`;
-// the solution x64 is accumulated in FP64.
+// the solution increments go to xinc when given (inner precision, flushed
+// into x64 by bl_flush), else straight into the FP64 solution x64.
 template <typename T>
 void bl_iteration(int dim, int nr, const BLArgs<T> &a, const T *invD, T *z, T *p, T *q, T *r,
-                  T *slots, double *x64, double *ppq, double *pnode, int parity, hipStream_t s);
+                  T *slots, double *x64, T *xinc, double *ppq, double *pnode, int parity,
+                  hipStream_t s);
+// x64 += xinc, xinc = 0 (every node of an instance with an unfinished system)
+template <typename T>
+void bl_flush(int dim, int nr, const BLArgs<T> &a, T *xinc, double *x64, hipStream_t s);
 // Residual replacement of the systems with state 1: y64 = K x64 (flagged
 // instances), r = f - y64, z = invD r, partials for the next iteration; then
 // state 1 -> 2.  count: unfinished systems.

@@ -374,7 +374,8 @@ struct BLUpdate {
   BLArgs<T> m;
   const T *q, *slots, *invD;
   T *p, *r, *z;
-  double *x;           // solution, accumulated in FP64
+  double *x;           // solution (FP64)
+  T *xinc;             // non-null: the increments go here (flushed into x by k_bl_flush)
   const double *ppq;
   double *prz, *prr;   // this iteration's partials [nblk][NR]
   int parity;
@@ -448,7 +449,10 @@ __global__ __launch_bounds__(BL_UPD_NODES * NR) void k_bl_update(BLUpdate<T> a) 
     const T al = T(s_al[v]);
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-      a.x[o + c] += s_al[v] * double(pv[c]);
+      if (a.xinc)
+        a.xinc[o + c] += al * pv[c];
+      else
+        a.x[o + c] += s_al[v] * double(pv[c]);
       const T rv = a.r[o + c] - al * qv[c];
       a.r[o + c] = rv;
       const T zv = a.invD[g * DIM + c] * rv;
@@ -461,6 +465,28 @@ __global__ __launch_bounds__(BL_UPD_NODES * NR) void k_bl_update(BLUpdate<T> a) 
   if (threadIdx.x < NR) {
     a.prz[int64_t(blockIdx.x) * NR + threadIdx.x] = rz;
     a.prr[int64_t(blockIdx.x) * NR + threadIdx.x] = rr;
+  }
+}
+
+// x += xinc, xinc = 0 for the nodes of a block (the increments of the
+// replay's iterations, accumulated in the inner precision); blocks whose
+// instance has finished every system are skipped (their xinc is zero: the
+// flush before the replacement that finished them emptied it).
+template <typename T, int DIM, int NR>
+__global__ __launch_bounds__(BL_UPD_NODES * NR) void k_bl_flush(BLArgs<T> m, T *xinc, double *x) {
+  const BLBlock B = m.blocks[blockIdx.x];
+  bool live = false;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) live |= m.st[int64_t(B.inst) * NR + r].state != 3;
+  if (!live) return;
+  const int v = threadIdx.x % NR;
+  const int64_t g = B.n0 + threadIdx.x / NR;
+  if (g >= B.n1) return;
+  const int64_t o = (g * NR + v) * DIM;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    x[o + c] += double(xinc[o + c]);
+    xinc[o + c] = T(0);
   }
 }
 
@@ -670,7 +696,8 @@ void bl_init_state(int n_state, BLState *st, hipStream_t s) {
 
 template <typename T>
 void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p, T *q, T *r,
-                  T *slots, double *x64, double *ppq, double *pnode, int parity, hipStream_t s) {
+                  T *slots, double *x64, T *xinc, double *ppq, double *pnode, int parity,
+                  hipStream_t s) {
   BLPatch<T> a{};
   a.m = m;
   a.x = z;
@@ -691,6 +718,7 @@ void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p
   u.r = r;
   u.z = z;
   u.x = x64;
+  u.xinc = xinc;
   u.ppq = ppq;
   u.prz = pnode + size_t(2 * parity + 0) * m.nblk * nr;
   u.prr = pnode + size_t(2 * parity + 1) * m.nblk * nr;
@@ -707,6 +735,22 @@ void bl_iteration(int dim, int nr, const BLArgs<T> &m, const T *invD, T *z, T *p
     else FEMGPU_BL_UPD(3, 4);
   }
 #undef FEMGPU_BL_UPD
+}
+
+template <typename T>
+void bl_flush(int dim, int nr, const BLArgs<T> &m, T *xinc, double *x64, hipStream_t s) {
+#define FEMGPU_BL_FL(D, R) \
+  hipLaunchKernelGGL((k_bl_flush<T, D, R>), dim3(m.nblk), dim3(BL_UPD_NODES * R), 0, s, m, xinc, x64)
+  if (dim == 2) {
+    if (nr == 1) FEMGPU_BL_FL(2, 1);
+    else if (nr == 2) FEMGPU_BL_FL(2, 2);
+    else FEMGPU_BL_FL(2, 4);
+  } else {
+    if (nr == 1) FEMGPU_BL_FL(3, 1);
+    else if (nr == 2) FEMGPU_BL_FL(3, 2);
+    else FEMGPU_BL_FL(3, 4);
+  }
+#undef FEMGPU_BL_FL
 }
 
 template <typename T, bool FINAL>
@@ -779,11 +823,12 @@ void bl_final(int dim, int nr, const BLArgs<double> &m64, const double *f, const
 template void bl_inv_diag<float>(int64_t, const double *, float *, hipStream_t);
 template void bl_inv_diag<double>(int64_t, const double *, double *, hipStream_t);
 template void bl_iteration<float>(int, int, const BLArgs<float> &, const float *, float *, float *,
-                                  float *, float *, float *, double *, double *, double *, int,
-                                  hipStream_t);
+                                  float *, float *, float *, double *, float *, double *, double *,
+                                  int, hipStream_t);
 template void bl_iteration<double>(int, int, const BLArgs<double> &, const double *, double *,
                                    double *, double *, double *, double *, double *, double *,
-                                   double *, int, hipStream_t);
+                                   double *, double *, int, hipStream_t);
+template void bl_flush<float>(int, int, const BLArgs<float> &, float *, double *, hipStream_t);
 template void bl_replace<float>(int, int, const BLArgs<float> &, const BLArgs<double> &,
                                 const double *, const double *, double *, double *, const float *,
                                 float *, float *, double *, int, int *, hipStream_t);
