@@ -44,7 +44,7 @@ struct BLState {
   double thr;          // tol^2 ||f||^2 (< 0: not yet known)
   double ff;           // ||f||^2 (FP64)
   double rr_last;      // FP64 r'r at the last replacement
-  double rr;           // last r'r seen by the stop test
+  double rr;           // last r'r seen by the stop test (while paused: the one at the pause)
   int32_t it[2];
   int32_t state, status, iters, nrep;
   int32_t restart;     // set with state 2: restart the directions (p = z)

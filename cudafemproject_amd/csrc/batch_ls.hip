@@ -181,7 +181,7 @@ __global__ __launch_bounds__(PT) void k_bl_patch(BLPatch<T> a) {
             S.thr = thr;
             S.ff = rrv;
           }
-          S.rr = rrv;
+          if (s != 1) S.rr = rrv;  // paused: keep the recurrence's r'r at the pause
           if (done) {
             if (s != 3) {
               S.state = 3;
@@ -569,7 +569,10 @@ __global__ __launch_bounds__(1024) void k_bl_mark(int n_inst, int nr, const BLIn
       const int r = i % nr;
       double rr = 0.0;
       for (int b = I.u0; b < I.u1; ++b) rr += prr[int64_t(b) * nr + r];
-      S.restart = rr > 0.25 * S.rr_last ? 1 : 0;
+      // restart the directions when the FP64 residual did not at least halve
+      // since the previous replacement, or when the recurrence had drifted
+      // far from it (FP64 residual more than twice the recurrence's)
+      S.restart = rr > 0.25 * S.rr_last || rr > 4.0 * S.rr ? 1 : 0;
       S.rr_last = rr;
       S.state = 2;
       ++S.nrep;

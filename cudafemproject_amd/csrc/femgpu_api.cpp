@@ -435,14 +435,19 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
 }
 
 // Replays the iteration graph until the device-side stop flag is set or,
-// when pause_rr > 0, until r'r has dropped below pause_rr.  Replays are
-// pipelined: replay k+1 is queued before the host looks at the scalars copied
-// after replay k (the kernels of a finished solve exit at once).  Every replay
-// either advances the iteration counter or sets the stop flag; anything else
-// means the kernels did not run.  On return the stream is idle and h_sc[0]
-// holds the last scalars.
+// when pause_host > 0, a replay ends with r'r below pause_host.  Replays are
+// pipelined: replay k+1 is queued before the host looks at the
+// scalars copied after replay k (the kernels of a finished solve exit at
+// once), except when the replay in flight is expected to stop -- its stop
+// metric (gamma or r'r against the threshold, r'r against the pause level),
+// extrapolated by the last replay's reduction factor, crosses the threshold,
+// or the iteration cap falls inside it: then the next replay is launched
+// only if the host sees it is still needed, which saves a whole replay of
+// no-op launches per stop.  Every replay either advances the iteration
+// counter or sets the stop flag; anything else means the kernels did not
+// run.  On return the stream is idle and h_sc[0] holds the last scalars.
 int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block, int max_iter,
-                 double pause_rr) {
+                 double pause_host = 0.0) {
   int st;
   const long max_replays = long(max_iter) / block + 3;
   auto launch = [&](int slot) -> int {
@@ -454,19 +459,38 @@ int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block,
   };
   if ((st = launch(0))) return st;
   int cur = 0, last_iter = -1;
+  bool seen = false;                  // a finished replay's scalars are known
+  double m_last = 0.0, m_rate = 1.0;  // stop metric and its reduction per replay
+  double r_last = 0.0, r_rate = 1.0;  // r'r and its reduction per replay
+  double thr = 0.0, pause = 0.0;
+  bool queued = false;  // replay nxt was launched before looking at replay cur
   for (long rep = 0;; ++rep) {
     const int nxt = (cur + 1) & 1;
-    if ((st = launch(nxt))) return st;
+    const bool near = seen && (m_last * m_rate <= thr || (pause > 0.0 && r_last * r_rate < pause) ||
+                               r_last * r_rate < pause_host || last_iter + block >= max_iter);
+    queued = !near;
+    if (queued && (st = launch(nxt))) return st;
     HIPCHK(hipEventSynchronize(ctx->ev_poll[cur]));
     const CgScalars &h = ctx->h_sc[cur];
-    if (h.done || (pause_rr > 0 && h.rr_final < pause_rr)) break;
+    if (h.done || (pause_host > 0 && h.rr_final < pause_host)) break;
     if (h.iter_final <= last_iter || rep > max_replays)
       return fail(ctx, FEMGPU_E_HIP, "PCG made no progress (kernels not running)");
     last_iter = h.iter_final;
+    const double m = h.kind == 0 ? h.g_final : h.rr_final;
+    if (seen && m_last > 0.0) m_rate = std::min(1.0, m / m_last);
+    if (seen && r_last > 0.0) r_rate = std::min(1.0, h.rr_final / r_last);
+    m_last = m;
+    r_last = h.rr_final;
+    thr = h.thresh;
+    pause = h.pause;
+    seen = true;
+    if (near && (st = launch(nxt))) return st;
     cur = nxt;
   }
+  // the state after the last replay run: the queued one (which iterated on
+  // after a host-side pause, or exited at once after a stop) when there is one
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  ctx->h_sc[0] = ctx->h_sc[(cur + 1) & 1];
+  ctx->h_sc[0] = ctx->h_sc[queued ? (cur + 1) & 1 : cur];
   return FEMGPU_OK;
 }
 
@@ -489,7 +513,7 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
   block = std::max(2, block + (block & 1));
   hipGraphExec_t exec;
   if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node, &exec)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0)))
+      (st = replay_until(ctx, exec, sc, block, max_iter)))
     return st;
   *iters = ctx->h_sc->iter_final;
   *done_code = ctx->h_sc->done;
@@ -571,7 +595,7 @@ int run_cg1(femgpu_ctx *ctx, femgpu::Cg1State<T> &w, const T *invD, int kind, do
   hipGraphExec_t exec;
   int st;
   if ((st = cg1_graph<T>(ctx, w, invD, block, prec_tag, sc, &exec)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0)))
+      (st = replay_until(ctx, exec, sc, block, max_iter)))
     return st;
   *pos = cg1_pos(*ctx->h_sc);
   *done_code = ctx->h_sc->done;
@@ -1125,13 +1149,13 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
     // residual has dropped by `drop`: run further, the FP32 recurrences of
     // s and w lose the accuracy the step-length formula relies on.
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0))) return st;
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter))) return st;
       done = ctx->h_sc->done;
       pos = cg1_pos(*ctx->h_sc);
       if (trace_enabled())
-        std::fprintf(stderr, "femgpu   inner stop: r'r %.6e thresh %.6e pause %.6e rr0 %.6e it %d/%d\n",
-                     ctx->h_sc->rr_final, ctx->h_sc->thresh, ctx->h_sc->pause, ctx->h_sc->rr0,
-                     ctx->h_sc->it[0], ctx->h_sc->it[1]);
+        std::fprintf(stderr, "femgpu   inner stop %d (parity %d): r'r %.6e thresh %.6e pause %.6e rr0 %.6e it %d/%d\n",
+                     ctx->h_sc->done, ctx->h_sc->par_final, ctx->h_sc->rr_final, ctx->h_sc->thresh,
+                     ctx->h_sc->pause, ctx->h_sc->rr0, ctx->h_sc->it[0], ctx->h_sc->it[1]);
       femgpu::launch_flush_x(ctx->dm, x64, w.x[pos.q], ctx->stream);
       if (done != 0 && done != 1 && done != 5) {
         status = done_status(done);
@@ -1254,11 +1278,14 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     // `inner_tol` (reliable updates), and at the end until the FP64 residual
     // meets the tolerance.
     PcgWs<float> w;
-    double *r64, *y64, *part64, *part_node, *part_patch;
+    double *r64, *y64, *part64, *part_node, *part_patch, *part_res;
+    // part_res: the FP64 residual's partials (apart from the inner
+    // iterations' part_node)
     if ((st = pcg_ws(ctx, "c32", w)) || (st = dbuf(ctx, "ir_r", nd, &r64)) ||
         (st = dbuf(ctx, "ir_y", nd, &y64)) ||
         (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part64)) ||
         (st = dbuf(ctx, "part_node", NODE_PART, &part_node)) ||
+        (st = dbuf(ctx, "part_res", size_t(femgpu::grid_blocks(ctx->dm.n_active)) + 1, &part_res)) ||
         (st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)))
       return st;
     w.invD = static_cast<float *>(ctx->bufs["invD32"].p);
@@ -1278,11 +1305,16 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
                                &exec)))
       return st;
     int status = FEMGPU_OK, replacements = 0;
-    double pause = ff * drop * drop;
     double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
+    // The host checks after every replay (of check_every iterations) whether
+    // the FP32 residual has dropped by `drop`; unless the drop was foreseen,
+    // the replay queued behind it keeps iterating meanwhile and the FP64
+    // residual is formed after that one.
+    double pause_rr = ff * drop * drop;
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause))) return st;
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause_rr))) return st;
       const int done = ctx->h_sc->done;
+      const double rr_rec = ctx->h_sc->rr_final;
       iters = ctx->h_sc->iter_final;
       femgpu::launch_flush_x(ctx->dm, x64, w.x, ctx->stream);
       if (done != 0 && done != 1) {
@@ -1292,23 +1324,30 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
       // FP64 residual of the accumulated solution
       femgpu::launch_reset_scalars(sc2, 1, 0.0, 0, ctx->stream);
       femgpu::launch_matvec_patch<double>(ctx->dm, x64, y64, part64, ctx->stream);
-      femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
+      femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_res, ctx->stream);
       HIPCHK(hipGetLastError());
       if ((st = poll(ctx, sc2))) return st;
       rel = ff > 0 ? std::sqrt(ctx->h_sc->rr / ff) : 0.0;
+      const double rel_rec = ff > 0 ? std::sqrt(rr_rec / ff) : 0.0;
+      if (trace_enabled())
+        std::fprintf(stderr, "femgpu mixed: %d iterations, stop %d, FP32 recurrence rel-res %.3e, "
+                     "FP64 rel-res %.3e\n", iters, done, rel_rec, rel);
       if (rel <= o.tol) break;
       if (iters >= max_iter || replacements > 1000) {
         status = FEMGPU_E_MAXIT;
         break;
       }
+      // Reliable update: r is replaced by the FP64 residual, the directions
+      // kept.  They are dropped (p = 0: plain refinement) when the recurrence
+      // had drifted far from the FP64 residual (more than twice it) or the
+      // FP64 residual did not at least halve since the previous replacement:
+      // then the directions belong to another residual.
       femgpu::launch_pcg_replace(ctx->dm, r64, w.r, w.z, w.invD, part_node, ctx->stream);
-      if (done == 1) femgpu::launch_pcg_resume(sc, ctx->stream);
-      // If the FP64 residual did not at least halve since the previous
-      // replacement, the FP32 directions have lost conjugacy: restart the
-      // Krylov space from the FP64 residual (p = 0), i.e. plain refinement.
-      if (rel > 0.5 * rel_prev) femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
+      if (rel > 2.0 * rel_rec || rel > 0.5 * rel_prev)
+        femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
+      if (done) femgpu::launch_pcg_resume(sc, ctx->stream);
       rel_prev = rel;
-      pause = ctx->h_sc->rr * drop * drop;
+      pause_rr = ctx->h_sc->rr * drop * drop;
       ++replacements;
     }
     S.iterations = iters;

@@ -56,7 +56,7 @@ struct CgScalars {
   double scale;        // sqrt(rr) (refinement scaling)
   int it[2];           // updates done before iteration j
   int iter_final;      // updates done when the stop flag was set
-  int par_final;       // single-reduction PCG: parity of the launch that set the stop flag
+  int par_final;       // parity of the launch that set the stop flag
   int kind;            // 0: gamma ratio, 1: relative residual
   int done;            // 0 running, 1 converged, 2 non-finite, 3 breakdown, 4 max-iter,
                        // 5 paused for a residual replacement (single-reduction PCG)
@@ -72,6 +72,8 @@ struct OpShift {
 };
 
 int lds_bytes(int dim, int elem_bytes, int pn, int ent);
+// blocks of the node-parallel kernels over n items (one partial per block)
+int grid_blocks(int64_t n);
 // Per-kernel launch attributes (large dynamic LDS); call once per device
 // before any launch or capture.
 hipError_t prepare_kernels();
@@ -112,6 +114,8 @@ void launch_residual64(const DevMesh &dm, const double *f, const double *y, cons
 void launch_flush_x(const DevMesh &dm, double *x64, float *x32, hipStream_t s);
 void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, float *z32,
                         const float *invD, double *part_node, hipStream_t s);
+// Clear the stop flag for the next replay (parity 0) after a residual
+// replacement: the scalars of the stopping launch move to slot 1.
 void launch_pcg_resume(CgScalars *sc, hipStream_t s);
 // gamma0 / thresholds from the slot-1 partials (after launch_pcg_replace).
 void launch_fin_init(CgScalars *sc, double *part_node, hipStream_t s);

@@ -136,6 +136,7 @@ void k_ebe_patch(PatchArgs<T> a) {
       sc->iter_final = n_upd;
       if (stop) {
         sc->done = stop;
+        sc->par_final = a.parity;
       } else {
         sc->g[a.parity] = g;
         sc->b[a.parity] = bt;
@@ -570,9 +571,14 @@ __global__ __launch_bounds__(UPT) void k_pcg_replace(int64_t n, const double *r6
 // Resume a stopped PCG at the next even iteration: the iteration that set
 // the stop flag had parity iter_final & 1 and left the last completed
 // iteration's gamma and count in the other slot; the next replay reads slot 1.
+// Resume the fused PCG after a stop (the residual replaced, its partials in
+// slot 1): the next replay starts at parity 0 and reads the scalars of slot
+// 1, which must be the last iteration's -- in slot 0 when the stopping
+// launch had parity 1 (recorded at the stop: after a resume at an odd
+// iteration the launch parity no longer follows the iteration count).
 __global__ void k_pcg_resume(CgScalars *sc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (sc->iter_final & 1) {
+    if (sc->par_final == 1) {
       sc->g[1] = sc->g[0];
       sc->b[1] = sc->b[0];
       sc->it[1] = sc->it[0];
@@ -947,6 +953,8 @@ hipError_t prepare_kernels() {
 #undef FEMGPU_ATTR
   return e;
 }
+
+int grid_blocks(int64_t n) { return grid_for(n); }
 
 int lds_bytes(int dim, int elem_bytes, int pn, int ent) {
   return lds_layout(dim, elem_bytes, pn, ent).total;

@@ -132,6 +132,33 @@ def test_mixed_vs_fp64(ctx, name):
     assert rel(u32, u64) < 1e-3
 
 
+@pytest.mark.parametrize("case", range(4))
+def test_mixed_iterations_on_plate_loads(case):
+    """Mixed precision on a 160 x 80-class plate of the batched workload
+    (instance 400 of plate_batch seed 2024) under its four load cases: the
+    FP32 inner iterations with FP64 residual replacement converge in at most
+    twice the FP64 iteration count.  (The body-force case once needed 24x:
+    the directions were kept after a replacement that found the FP64
+    residual twice the FP32 recurrence's.)"""
+    from cudafemproject_amd.problem import plate_batch
+    b = plate_batch(401, seed=2024)[400]
+    c = F.Context(0)
+    try:
+        c.set_mesh(b.nodes, b.elems)
+        c.set_material(b.E, b.nu)
+        c.set_constraints(b.constr_node, b.constr_code)
+        f = b.f[:, case, :]
+        u64, s64 = c.solve_rhs(f, F.FP64, tol=1e-8)
+        um, sm = c.solve_rhs(f, F.MIXED, tol=1e-8, max_iter=20 * s64["iterations"])
+        print(f"[plate mixed] load {case}: it {sm['iterations']} (fp64 {s64['iterations']}), "
+              f"outer {sm['outer_iterations']}", flush=True)
+        assert sm["status"] == 0 and sm["rel_res"] <= 1e-8 * 1.0001
+        assert rel(um, u64) < 1e-7
+        assert sm["iterations"] <= 2 * s64["iterations"]
+    finally:
+        c.close()
+
+
 def test_deterministic(ctx):
     pb = PROBLEMS["cube"]
     ctx.set_problem(pb)
