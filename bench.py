@@ -184,7 +184,7 @@ def bench_column(args, F):
         "fp32_vs_fp64_rel_displacement_diff": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "one explicit step fp64 (predict + EbE matvec + correct)",
+                     "kernel": "one explicit step fp64 (k_ebe_patch MODE 4: predictor, EbE matvec, corrector)",
                      "bytes_per_launch": b64},
         "fp32_step_gbs": b32 / (us32 * 1e-6) / 1e9,
         "cpu_baseline": None,

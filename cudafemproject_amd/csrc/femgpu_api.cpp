@@ -1720,6 +1720,51 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   // explicit steps: the displacement alternates between two buffers (the
   // fused predictor reads one and writes the other); step n writes dbufs[n & 1 ^ 1]
   T *dbufs[2] = {dd, dd2};
+  // Without receivers or relaxation (which read v and a every step) a step
+  // is one patch launch (MODE 4); v, a and the interface slots then
+  // alternate between two buffers as well, and the interface nodes' v and a
+  // lag one step (the displacement does not).
+  const bool one_launch = expl && nrec == 0 && !relax;
+  T *vbufs[2] = {dv, dv}, *abufs[2] = {da, da}, *pbufs[2] = {dpart, dpart};
+  if (one_launch) {
+    T *dv2, *da2, *dpart2;
+    if ((st = dbuf(ctx, tg + "v2", nd, &dv2)) || (st = dbuf(ctx, tg + "a2", nd, &da2)) ||
+        (st = dbuf(ctx, tg + "part2", size_t(ctx->plan.n_slots) * d + 1, &dpart2)))
+      return st;
+    femgpu::launch_fill<T>(dv2, int64_t(nd), T(0), ctx->stream);
+    femgpu::launch_fill<T>(da2, int64_t(nd), T(0), ctx->stream);
+    vbufs[1] = dv2;
+    abufs[1] = da2;
+    pbufs[1] = dpart2;
+  }
+  // one explicit step at buffer parity par (k: the step's offset from dstep)
+  auto explicit_step = [&](int par, int k) {
+    if (one_launch) {
+      femgpu::DynStep<T> ds{};
+      ds.d_in = dbufs[par];
+      ds.d_out = dbufs[par ^ 1];
+      ds.v_in = vbufs[par];
+      ds.v_out = vbufs[par ^ 1];
+      ds.a_in = abufs[par];
+      ds.a_out = abufs[par ^ 1];
+      ds.part_in = pbufs[par ^ 1];
+      ds.part_out = pbufs[par];
+      ds.f_file = dff;
+      ds.f_unit = dfu;
+      ds.invM = dinvM;
+      ds.coef_file = dcf;
+      ds.coef_unit = dcu;
+      ds.step_base = dstep;
+      ds.step_off = k;
+      femgpu::launch_dyn_step<T>(ctx->dm, ds, C, ctx->stream);
+    } else {
+      femgpu::launch_dyn_predict_matvec<T>(ctx->dm, dbufs[par], dv, da, dbufs[par ^ 1], C, dy,
+                                           dpart, ctx->stream);
+      femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM,
+                                    da, dv, nullptr, C, ctx->stream, nullptr, true);
+      record(k, dbufs[par ^ 1]);
+    }
+  };
   if (expl) {
     // Steps are captured in a HIP graph of G steps (G even, so every replay
     // starts on buffer 0) when at least one full graph fits the run; the
@@ -1732,13 +1777,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
       hipGraphExec_t exec = nullptr;
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < G; ++k) {
-        femgpu::launch_dyn_predict_matvec<T>(ctx->dm, dbufs[k & 1], dv, da, dbufs[(k & 1) ^ 1], C,
-                                             dy, dpart, ctx->stream);
-        femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM,
-                                      da, dv, nullptr, C, ctx->stream, nullptr, true);
-        record(k, dbufs[(k & 1) ^ 1]);
-      }
+      for (int k = 0; k < G; ++k) explicit_step(k & 1, k);
       HIPCHK(hipStreamEndCapture(ctx->stream, &g));
       const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -1767,12 +1806,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     }
     femgpu::launch_set_int(dstep, done + 1, ctx->stream);
     for (int k = 0; done < endnt && !settled; ++k, ++done) {
-      const int par = done & 1;
-      femgpu::launch_dyn_predict_matvec<T>(ctx->dm, dbufs[par], dv, da, dbufs[par ^ 1], C, dy,
-                                           dpart, ctx->stream);
-      femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, k, dinvM, da,
-                                    dv, nullptr, C, ctx->stream, nullptr, true);
-      record(k, dbufs[par ^ 1]);
+      explicit_step(done & 1, k);
       if (relax) {
         double vm = 0.0;
         if ((st = vmax(&vm))) return st;

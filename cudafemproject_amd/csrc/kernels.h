@@ -152,6 +152,22 @@ void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T
                         const int *step_base, int step_off, const T *invM, T *a, T *v,
                         T *r_out, DynCoeffs c, hipStream_t s, const T *mlump = nullptr,
                         bool pred_v = false);
+
+// Buffers of one explicit GN22 step in a single patch launch (MODE 4): d, v,
+// a and the interface slots each alternate between two buffers; the
+// interface nodes' v and a lag one step (corrected at the next launch; the
+// displacement of every node is current after each launch).
+template <typename T>
+struct DynStep {
+  const T *d_in, *v_in, *a_in, *part_in;
+  T *d_out, *v_out, *a_out, *part_out;
+  const T *f_file, *f_unit, *invM;
+  const double *coef_file, *coef_unit;
+  const int *step_base;
+  int step_off;
+};
+template <typename T>
+void launch_dyn_step(const DevMesh &dm, const DynStep<T> &st, DynCoeffs c, hipStream_t s);
 // Explicit GN22 predictor fused with the matvec: d_out = (d_in + dt v) + c1 a
 // (owned and first-slot nodes), y / partial = K d_out.
 template <typename T>

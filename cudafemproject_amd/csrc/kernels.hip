@@ -44,7 +44,11 @@ using namespace patch;
 // Patch kernel.  MODE 0: y = K x.  MODE 1: fused PCG step (stop test and beta
 // from the previous update's partial sums, p = z + beta p, q = cK K p + cM m p,
 // per-patch p'q partial).  MODE 2: diag(K).  MODE 3: explicit GN22 step, the
-// predictor d~ = d + dt v + c1 a fused with y = K d~.
+// predictor d~ = d + dt v + c1 a fused with y = K d~.  MODE 4: the whole
+// explicit step in one launch: MODE 3, then the corrector of the owned nodes
+// from their complete K d~; an interface node is corrected one launch late,
+// by every patch that holds it, from the previous launch's slots (the first
+// patch stores the result).
 
 template <typename T>
 struct PatchArgs {
@@ -71,10 +75,31 @@ struct PatchArgs {
   const double *prz;  // MODE 1: r'z partials of the previous update (NODE_PARTS)
   const double *prr;  // MODE 1: r'r partials of the previous update
   int parity;         // MODE 1: j & 1
-  const T *vel, *acc; // MODE 3: velocity and acceleration of the previous step
-  T *xout;            // MODE 3: predicted displacement (owned and first-slot nodes)
-  T dt, c1;           // MODE 3: GN22 predictor coefficients
+  const T *vel, *acc; // MODE 3/4: velocity and acceleration of the previous step
+  T *xout;            // MODE 3/4: predicted displacement (owned and first-slot nodes)
+  T dt, c1;           // MODE 3/4: GN22 predictor coefficients
+  // MODE 4 (for interface nodes vel/acc and vout/aout lag one step)
+  T *vout, *aout;           // corrected velocity and acceleration
+  const T *prev_part;       // the previous launch's interface slots
+  const T *f_file, *f_unit; // load vectors and their per-step coefficients
+  const double *coef_file, *coef_unit;
+  const int *step_base;     // step = *step_base + step_off (1-based)
+  int step_off;
+  const T *invM;            // lumped inverse mass
+  T c2, c3, alpha, scale;   // GN22 corrector and mass-damping coefficients
 };
+
+// GN22 corrector of one DOF (the operations of k_dyn_correct): r = F - K d~,
+// v~ = v + c2 a, (1 + alpha c3) M a' = r - alpha M v~, v' = v~ + c3 a'.
+template <typename T>
+__device__ __forceinline__ void gn22_correct(T cf, T cu, T ff, T fu, T kd, T im, T c2, T c3,
+                                             T alpha, T scale, T &a, T &v) {
+  const T r = cf * ff + cu * fu - kd;
+  const T vp = v + c2 * a;
+  const T av = alpha != T(0) ? (r * im - alpha * vp) * scale : r * im;
+  a = av;
+  v = vp + c3 * av;
+}
 
 // One workgroup per patch, one patch node per thread (phases A and C), up to
 // EPT elements per thread (phase B).  The element forces stay in registers
@@ -182,6 +207,12 @@ void k_ebe_patch(PatchArgs<T> a) {
   T xin[DIM];
 #pragma unroll
   for (int c = 0; c < DIM; ++c) xin[c] = T(0);
+  // MODE 4: the node's acceleration and velocity entering this step, and
+  // whether this patch holds the node's first slot
+  T av4[DIM], vv4[DIM];
+  bool first = own;
+  int step4 = 0;
+  if constexpr (MODE == 4) step4 = *a.step_base + a.step_off;
   if (has_node) {
     gid = own ? int64_t(P.own_start + k) : int64_t(a.sh_gid[P.sh_start + k - P.n_own]);
     if (!own) slot = a.sh_slot[P.sh_start + k - P.n_own];
@@ -209,6 +240,34 @@ void k_ebe_patch(PatchArgs<T> a) {
         const T dv = a.x[gid * DIM + c] + a.dt * a.vel[gid * DIM + c];
         xin[c] = dv + a.c1 * a.acc[gid * DIM + c];
       }
+    } else if constexpr (MODE == 4) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        av4[c] = a.acc[gid * DIM + c];
+        vv4[c] = a.vel[gid * DIM + c];
+      }
+      if (!own) {
+        const int64_t s = gid - a.n_owned;
+        const int s0 = a.slot_start[s], s1 = a.slot_start[s + 1];
+        first = slot == s0;
+        if (step4 > 1) {
+          // the previous step of this interface node, from its slots
+          T kd[DIM];
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) kd[c] = T(0);
+          add_slots<T, DIM>(a.prev_part, s0, s1, kd);
+          const T cf = T(a.coef_file[step4 - 1]), cu = T(a.coef_unit[step4 - 1]);
+#pragma unroll
+          for (int c = 0; c < DIM; ++c)
+            gn22_correct<T>(cf, cu, a.f_file[gid * DIM + c], a.f_unit[gid * DIM + c], kd[c],
+                            a.invM[gid * DIM + c], a.c2, a.c3, a.alpha, a.scale, av4[c], vv4[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const T dv = a.x[gid * DIM + c] + a.dt * vv4[c];
+        xin[c] = dv + a.c1 * av4[c];
+      }
     }
     if constexpr (MODE != 2) {
 #pragma unroll
@@ -218,7 +277,6 @@ void k_ebe_patch(PatchArgs<T> a) {
   // interface nodes: does this patch hold the node's first slot?  (The
   // first patch writes the constrained rows and, in the PCG step, adds the
   // node's mass term of p'(cK K + cM M) p.)
-  bool first = own;
   if constexpr (MODE == 1) {
     if (has_node && !own && (a.sh.m || fm)) first = slot == a.slot_start[gid - a.n_owned];
     if (has_node && !own && first && a.sh.m) {
@@ -254,7 +312,26 @@ void k_ebe_patch(PatchArgs<T> a) {
           if ((fm >> c) & 1u) acc[c] = first ? a.diagK[gid * DIM + c] * xin[c] : T(0);
       }
     }
-    if (own) {
+    if constexpr (MODE == 4) {
+      if (own) {
+        const T cf = T(a.coef_file[step4]), cu = T(a.coef_unit[step4]);
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          gn22_correct<T>(cf, cu, a.f_file[gid * DIM + c], a.f_unit[gid * DIM + c], acc[c],
+                          a.invM[gid * DIM + c], a.c2, a.c3, a.alpha, a.scale, av4[c], vv4[c]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = acc[c];
+      }
+      if (first) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          a.aout[gid * DIM + c] = av4[c];
+          a.vout[gid * DIM + c] = vv4[c];
+          a.xout[gid * DIM + c] = xin[c];
+        }
+      }
+    } else if (own) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         T v = acc[c];
@@ -944,7 +1021,8 @@ hipError_t prepare_kernels() {
   FEMGPU_ATTR(T, D, 0, 256) FEMGPU_ATTR(T, D, 1, 256) FEMGPU_ATTR(T, D, 2, 256)                \
   FEMGPU_ATTR(T, D, 3, 256) FEMGPU_ATTR(T, D, 0, 512) FEMGPU_ATTR(T, D, 1, 512)                \
   FEMGPU_ATTR(T, D, 2, 512) FEMGPU_ATTR(T, D, 3, 512) FEMGPU_ATTR(T, D, 0, 1024)               \
-  FEMGPU_ATTR(T, D, 1, 1024) FEMGPU_ATTR(T, D, 2, 1024) FEMGPU_ATTR(T, D, 3, 1024)
+  FEMGPU_ATTR(T, D, 1, 1024) FEMGPU_ATTR(T, D, 2, 1024) FEMGPU_ATTR(T, D, 3, 1024)          \
+  FEMGPU_ATTR(T, D, 4, 256) FEMGPU_ATTR(T, D, 4, 512) FEMGPU_ATTR(T, D, 4, 1024)
   FEMGPU_ATTR_TD(double, 3)
   FEMGPU_ATTR_TD(double, 2)
   FEMGPU_ATTR_TD(float, 3)
@@ -1187,6 +1265,36 @@ void launch_dyn_correct(const DevMesh &dm, const T *y, const T *partial, const T
 }
 
 template <typename T>
+void launch_dyn_step(const DevMesh &dm, const DynStep<T> &st, DynCoeffs c, hipStream_t s) {
+  PatchArgs<T> a = patch_args<T>(dm);
+  a.x = st.d_in;
+  a.vel = st.v_in;
+  a.acc = st.a_in;
+  a.xout = st.d_out;
+  a.vout = st.v_out;
+  a.aout = st.a_out;
+  a.partial = st.part_out;
+  a.prev_part = st.part_in;
+  a.f_file = st.f_file;
+  a.f_unit = st.f_unit;
+  a.coef_file = st.coef_file;
+  a.coef_unit = st.coef_unit;
+  a.step_base = st.step_base;
+  a.step_off = st.step_off;
+  a.invM = st.invM;
+  a.dt = T(c.dt);
+  a.c1 = T(c.c1);
+  a.c2 = T(c.c2);
+  a.c3 = T(c.c3);
+  a.alpha = T(c.alpha);
+  a.scale = T(c.expl_scale);
+  if (dm.dim == 3)
+    launch_patch<T, 3, 4>(dm, a, s);
+  else
+    launch_patch<T, 2, 4>(dm, a, s);
+}
+
+template <typename T>
 void launch_dyn_predict_matvec(const DevMesh &dm, const T *d_in, const T *v, const T *acc,
                                T *d_out, DynCoeffs c, T *y, T *partial, hipStream_t s) {
   PatchArgs<T> a = patch_args<T>(dm);
@@ -1261,6 +1369,7 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
                                       const T *, const double *, const double *, const int *,  \
                                       int, const T *, T *, T *, T *, DynCoeffs, hipStream_t,   \
                                       const T *, bool);                                        \
+  template void launch_dyn_step<T>(const DevMesh &, const DynStep<T> &, DynCoeffs, hipStream_t); \
   template void launch_dyn_predict_matvec<T>(const DevMesh &, const T *, const T *, const T *, \
                                              T *, DynCoeffs, T *, T *, hipStream_t);          \
   template void launch_dyn_implicit_post<T>(const DevMesh &, const T *, T *, T *, DynCoeffs,   \

@@ -51,6 +51,32 @@ def test_traces_equal_separate_runs(ctx):
         np.testing.assert_allclose(tr[k - 1], u[recv], rtol=1e-13, atol=1e-30)
 
 
+@pytest.mark.parametrize("alpha", [0.0, 30.0])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", ["column", "plate"])
+def test_one_launch_step_matches_two_launch_step(ctx, shape, precision, alpha):
+    """Without receivers an explicit step is one patch launch whose interface
+    nodes are corrected one launch late; with receivers it is the predictor
+    launch plus the corrector launch.  Both give the same displacements
+    (meshes of 12 and 6 patches, so interface nodes are involved; 150 steps:
+    two graph replays and an eager tail; with and without mass damping)."""
+    pb = (P.column_problem(8, 8, 24, 0.4, 0.4, 1.2, convention=F.LOADS_PHYSICAL, pressure=1.0)
+          if shape == "column" else P.plate_problem(64, 32, convention=F.LOADS_PHYSICAL))
+    ctx.set_problem(pb)
+    n = 150
+    kw = dict(KW, precision=F.FP64 if precision == "fp64" else F.FP32, damping_alpha=alpha,
+              endtime=(n + 0.5) * KW["dt"])
+    u1, st1 = ctx.dyn_run(**kw)
+    ctx.set_receivers(np.array([0], np.int32), F.RECV_VELOCITY, 1)
+    try:
+        u2, st2 = ctx.dyn_run(**kw)
+    finally:
+        ctx.set_receivers([], F.RECV_DISPLACEMENT, 1)
+    assert st1["steps"] == st2["steps"] == n
+    assert np.linalg.norm(u2) > 0
+    assert rel(u1, u2) < (1e-12 if precision == "fp64" else 1e-5)
+
+
 def test_velocity_acceleration_central_differences(ctx):
     """GN22 with beta1 = 1/2, beta2 = 0: v_n = (d_{n+1} - d_{n-1}) / (2 dt) and
     a_n = (d_{n+1} - 2 d_n + d_{n-1}) / dt^2 exactly (to rounding)."""
