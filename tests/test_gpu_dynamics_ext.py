@@ -77,6 +77,29 @@ def test_one_launch_step_matches_two_launch_step(ctx, shape, precision, alpha):
     assert rel(u1, u2) < (1e-12 if precision == "fp64" else 1e-5)
 
 
+@pytest.mark.parametrize("tuning", [(1536, 512), (3072, 1024)])
+def test_one_launch_step_other_tunings(ctx, tuning):
+    """The one-launch explicit step with 512- and 1024-thread patches (the
+    FP64 kernels of the larger workgroups spill registers) against the
+    two-launch step."""
+    pb = P.column_problem(8, 8, 48, 0.4, 0.4, 2.4, convention=F.LOADS_PHYSICAL, pressure=1.0)
+    ctx.set_tuning(*tuning)
+    try:
+        ctx.set_problem(pb)
+        n = 70
+        kw = dict(KW, endtime=(n + 0.5) * KW["dt"])
+        u1, _ = ctx.dyn_run(**kw)
+        ctx.set_receivers(np.array([0], np.int32), F.RECV_DISPLACEMENT, 1)
+        try:
+            u2, _ = ctx.dyn_run(**kw)
+        finally:
+            ctx.set_receivers([], F.RECV_DISPLACEMENT, 1)
+    finally:
+        ctx.set_tuning(0, 0)
+    assert np.linalg.norm(u2) > 0
+    assert rel(u1, u2) < 1e-12
+
+
 def test_velocity_acceleration_central_differences(ctx):
     """GN22 with beta1 = 1/2, beta2 = 0: v_n = (d_{n+1} - d_{n-1}) / (2 dt) and
     a_n = (d_{n+1} - 2 d_n + d_{n-1}) / dt^2 exactly (to rounding)."""
