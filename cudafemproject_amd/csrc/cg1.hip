@@ -311,7 +311,8 @@ __global__ __launch_bounds__(DOT_THREADS) void k_cg1_dots(int64_t n, const T *r,
 
 // Scalars after an INIT launch.  mode 0: fresh start (g0, thresholds,
 // counters); 1: residual replacement keeping the search direction p, state
-// moved from parity q; 2: replacement with a restart (b = 0 next).
+// moved from parity q; 2: replacement with a restart (b = 0 next), also
+// after a breakdown stop (done = 3).
 // Mode 1 sets the stored previous step length so that the next launch's
 // denominator d - b g / a_prev equals p'A p of the new direction
 // p = u + b p_prev exactly: p'A p = d + 2 b s_prev'u + b^2 p_prev's_prev
@@ -366,7 +367,8 @@ __global__ void k_cg1_fin(CgScalars *sc, const double *part, int np, int mode, i
     const double pap = d + 2.0 * bt * su + bt * bt * ps;
     sc->al[1] = (bt == 0.0 || pap == d) ? inf : bt * g / (d - pap);
   }
-  if (sc->done == 1 || sc->done == 5) sc->done = 0;
+  // a restart also clears a breakdown stop (the host restarts after one)
+  if (sc->done == 1 || sc->done == 5 || (mode == 2 && sc->done == 3)) sc->done = 0;
 }
 
 template <typename T>

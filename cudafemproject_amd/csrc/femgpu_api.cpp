@@ -434,57 +434,114 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
   return FEMGPU_OK;
 }
 
+// Stop prediction of replay_until: the stop metric (gamma or r'r, whichever
+// the stop test uses) and r'r of the last replays seen, their log reduction
+// per iteration over a window of at least one full replay (r'r of CG is not
+// monotone: shorter windows predict poorly), and the device's thresholds.
+// A caller that continues the same iteration counter after a pause (the
+// mixed solve after a residual replacement) passes it back in, reseeded
+// with the new residual; the rates carry over.
+struct ReplayPredict {
+  static constexpr int W = 8;
+  int n = 0;                        // replays in the window
+  int it[W];                        // their iteration counts, oldest first
+  double m[W], r[W];
+  double lm = 0.0, lr = 0.0;        // log reduction per iteration
+  double thr = 0.0, pause = 0.0;
+  bool seen() const { return n > 0; }
+  int iter() const { return n ? it[n - 1] : -1; }
+  double m_last() const { return m[n - 1]; }
+  double r_last() const { return r[n - 1]; }
+  void seed(int iter, double m0, double r0) {
+    n = 1;
+    it[0] = iter;
+    m[0] = m0;
+    r[0] = r0;
+  }
+  void add(int iter, double mv, double rv, int span) {
+    if (n == W) {
+      for (int k = 1; k < W; ++k) {
+        it[k - 1] = it[k];
+        m[k - 1] = m[k];
+        r[k - 1] = r[k];
+      }
+      --n;
+    }
+    it[n] = iter;
+    m[n] = mv;
+    r[n] = rv;
+    ++n;
+    int j = n - 2;
+    while (j > 0 && iter - it[j] < span) --j;
+    if (j < 0) return;
+    const double dn = double(iter - it[j]);
+    if (m[j] > 0.0 && mv > 0.0) lm = std::min(0.0, std::log(mv / m[j]) / dn);
+    if (r[j] > 0.0 && rv > 0.0) lr = std::min(0.0, std::log(rv / r[j]) / dn);
+  }
+};
+
 // Replays the iteration graph until the device-side stop flag is set or,
 // when pause_host > 0, a replay ends with r'r below pause_host.  Replays are
 // pipelined: replay k+1 is queued before the host looks at the
 // scalars copied after replay k (the kernels of a finished solve exit at
-// once), except when the replay in flight is expected to stop -- its stop
-// metric (gamma or r'r against the threshold, r'r against the pause level),
-// extrapolated by the last replay's reduction factor, crosses the threshold,
-// or the iteration cap falls inside it: then the next replay is launched
-// only if the host sees it is still needed, which saves a whole replay of
-// no-op launches per stop.  Every replay either advances the iteration
-// counter or sets the stop flag; anything else means the kernels did not
-// run.  On return the stream is idle and h_sc[0] holds the last scalars.
+// once, but each such launch still costs a few microseconds), except when
+// the replay in flight is expected to stop -- its stop metric (gamma or r'r
+// against the threshold, r'r against the pause level) extrapolated at the
+// last replays' rate crosses the threshold, or the iteration cap falls
+// inside it: then the next replay is launched only if the host sees it is
+// still needed.  A replay expected to contain the device stop runs the
+// short graph `tail` (tail_block iterations) when there is one, which cuts
+// the no-op launches after the stop from up to block - 1 to about
+// tail_block (fewer when the stop is foreseen).  Every replay
+// either advances the iteration counter or sets the stop flag; anything
+// else means the kernels did not run.  On return the stream is idle and
+// h_sc[0] holds the last scalars.
 int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block, int max_iter,
-                 double pause_host = 0.0) {
+                 double pause_host = 0.0, hipGraphExec_t tail = nullptr, int tail_block = 0,
+                 ReplayPredict *pred = nullptr) {
   int st;
-  const long max_replays = long(max_iter) / block + 3;
-  auto launch = [&](int slot) -> int {
-    HIPCHK(hipGraphLaunch(exec, ctx->stream));
+  if (!tail) tail_block = block;
+  const long max_replays = long(max_iter) / std::min(block, tail_block) + 3;
+  ReplayPredict local;
+  ReplayPredict &P = pred ? *pred : local;
+  int n_slot[2] = {0, 0};  // iterations of the replay in each slot
+  auto launch = [&](int slot, int n) -> int {
+    HIPCHK(hipGraphLaunch(n == block ? exec : tail, ctx->stream));
     HIPCHK(hipMemcpyAsync(&ctx->h_sc[slot], sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev_poll[slot], ctx->stream));
+    n_slot[slot] = n;
     return FEMGPU_OK;
   };
-  if ((st = launch(0))) return st;
-  int cur = 0, last_iter = -1;
-  bool seen = false;                  // a finished replay's scalars are known
-  double m_last = 0.0, m_rate = 1.0;  // stop metric and its reduction per replay
-  double r_last = 0.0, r_rate = 1.0;  // r'r and its reduction per replay
-  double thr = 0.0, pause = 0.0;
+  // the device stop within n more iterations, as far as the host can tell
+  auto stop_within = [&](int n) {
+    return P.seen() && (P.m_last() * std::exp(P.lm * n) <= P.thr ||
+                        (P.pause > 0.0 && P.r_last() * std::exp(P.lr * n) < P.pause) ||
+                        P.iter() + n >= max_iter);
+  };
+  auto pause_within = [&](int n) {
+    return P.seen() && pause_host > 0.0 && P.r_last() * std::exp(P.lr * n) < pause_host;
+  };
+  // short replays from twice the horizon on: they cost a copy of the
+  // scalars each, a full replay past the stop up to block - 1 no-op launches
+  auto size_for = [&](int horizon) { return stop_within(2 * horizon) ? tail_block : block; };
+  if ((st = launch(0, size_for(block)))) return st;
+  int cur = 0;
   bool queued = false;  // replay nxt was launched before looking at replay cur
   for (long rep = 0;; ++rep) {
     const int nxt = (cur + 1) & 1;
-    const bool near = seen && (m_last * m_rate <= thr || (pause > 0.0 && r_last * r_rate < pause) ||
-                               r_last * r_rate < pause_host || last_iter + block >= max_iter);
+    const bool near = stop_within(n_slot[cur]) || pause_within(n_slot[cur]);
     queued = !near;
-    if (queued && (st = launch(nxt))) return st;
+    if (queued && (st = launch(nxt, size_for(n_slot[cur] + block)))) return st;
     HIPCHK(hipEventSynchronize(ctx->ev_poll[cur]));
     const CgScalars &h = ctx->h_sc[cur];
     if (h.done || (pause_host > 0 && h.rr_final < pause_host)) break;
-    if (h.iter_final <= last_iter || rep > max_replays)
+    if (h.iter_final <= P.iter() || rep > max_replays)
       return fail(ctx, FEMGPU_E_HIP, "PCG made no progress (kernels not running)");
-    last_iter = h.iter_final;
-    const double m = h.kind == 0 ? h.g_final : h.rr_final;
-    if (seen && m_last > 0.0) m_rate = std::min(1.0, m / m_last);
-    if (seen && r_last > 0.0) r_rate = std::min(1.0, h.rr_final / r_last);
-    m_last = m;
-    r_last = h.rr_final;
-    thr = h.thresh;
-    pause = h.pause;
-    seen = true;
-    if (near && (st = launch(nxt))) return st;
+    P.add(h.iter_final, h.kind == 0 ? h.g_final : h.rr_final, h.rr_final, block);
+    P.thr = h.thresh;
+    P.pause = h.pause;
+    if (near && (st = launch(nxt, size_for(block)))) return st;
     cur = nxt;
   }
   // the state after the last replay run: the queued one (which iterated on
@@ -493,6 +550,9 @@ int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block,
   ctx->h_sc[0] = ctx->h_sc[queued ? (cur + 1) & 1 : cur];
   return FEMGPU_OK;
 }
+
+// Iterations of the short graph that replays expected to contain the stop run.
+inline int tail_block_of(int block) { return std::max(2, (block / 4) & ~1); }
 
 // Runs PCG until the stop flag is set.  r holds the initial residual; x and p
 // are cleared here.
@@ -511,9 +571,11 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
   femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.z, w.invD, sc, part_node, ctx->stream);
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
-  hipGraphExec_t exec;
+  const int tb = tail_block_of(block);
+  hipGraphExec_t exec, tail;
   if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node, &exec)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter)))
+      (st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)) ||
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, tail, tb)))
     return st;
   *iters = ctx->h_sc->iter_final;
   *done_code = ctx->h_sc->done;
@@ -592,10 +654,12 @@ int run_cg1(femgpu_ctx *ctx, femgpu::Cg1State<T> &w, const T *invD, int kind, do
   femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, nullptr, 0.0, ctx->stream);
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
-  hipGraphExec_t exec;
+  const int tb = tail_block_of(block);
+  hipGraphExec_t exec, tail;
   int st;
   if ((st = cg1_graph<T>(ctx, w, invD, block, prec_tag, sc, &exec)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter)))
+      (st = cg1_graph<T>(ctx, w, invD, tb, prec_tag, sc, &tail)) ||
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, tail, tb)))
     return st;
   *pos = cg1_pos(*ctx->h_sc);
   *done_code = ctx->h_sc->done;
@@ -1141,15 +1205,19 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
     HIPCHK(hipGetLastError());
     if ((st = poll(ctx, sc))) return st;
     const double ff = ctx->h_sc->rr0;
-    hipGraphExec_t exec;
-    if ((st = cg1_graph<float>(ctx, w, invD, block_m, 3, sc, &exec))) return st;
+    const int tb = tail_block_of(block_m);
+    hipGraphExec_t exec, tail;
+    if ((st = cg1_graph<float>(ctx, w, invD, block_m, 3, sc, &exec)) ||
+        (st = cg1_graph<float>(ctx, w, invD, tb, 3, sc, &tail)))
+      return st;
     int status = FEMGPU_OK, replacements = 0;
+    bool kept_p = false;  // the last replacement kept the search direction
     double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
     // The iterations pause on the device (done = 5) as soon as the FP32
     // residual has dropped by `drop`: run further, the FP32 recurrences of
     // s and w lose the accuracy the step-length formula relies on.
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter))) return st;
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0, tail, tb))) return st;
       done = ctx->h_sc->done;
       pos = cg1_pos(*ctx->h_sc);
       if (trace_enabled())
@@ -1157,7 +1225,11 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
                      ctx->h_sc->done, ctx->h_sc->par_final, ctx->h_sc->rr_final, ctx->h_sc->thresh,
                      ctx->h_sc->pause, ctx->h_sc->rr0, ctx->h_sc->it[0], ctx->h_sc->it[1]);
       femgpu::launch_flush_x(ctx->dm, x64, w.x[pos.q], ctx->stream);
-      if (done != 0 && done != 1 && done != 5) {
+      // a breakdown right after a replacement that kept p: the kept
+      // direction did not suit the new residual; restart from the FP64
+      // residual (the state is the last completed iteration's)
+      const bool retry = done == 3 && kept_p;
+      if (done != 0 && done != 1 && done != 5 && !retry) {
         status = done_status(done);
         break;
       }
@@ -1179,7 +1251,8 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
       // Restart (p = s = 0) when the FP64 residual did not at least halve
       // since the previous replacement: the FP32 directions have lost
       // conjugacy.  Otherwise keep p (moved to index 0) with s = A p.
-      const bool restart = rel > 0.5 * rel_prev;
+      const bool restart = rel > 0.5 * rel_prev || retry;
+      kept_p = !restart;
       femgpu::launch_fill<float>(w.x[0], nd, 0.f, ctx->stream);
       if (restart) {
         femgpu::launch_fill<float>(w.p[0], nd, 0.f, ctx->stream);
@@ -1300,11 +1373,18 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     HIPCHK(hipGetLastError());
     if ((st = poll(ctx, sc))) return st;
     const double ff = ctx->h_sc->rr0;
-    hipGraphExec_t exec;
+    const int tb = tail_block_of(block_m);
+    hipGraphExec_t exec, tail;
     if ((st = pcg_graph<float>(ctx, w, OpShift<float>(), block_m, 1, sc, part_patch, part_node,
-                               &exec)))
+                               &exec)) ||
+        (st = pcg_graph<float>(ctx, w, OpShift<float>(), tb, 1, sc, part_patch, part_node,
+                               &tail)))
       return st;
+    // the stop prediction carries over replacements: same iteration
+    // counter, the rate of the previous inner solve, the new residual
+    ReplayPredict pred;
     int status = FEMGPU_OK, replacements = 0;
+    bool kept_p = false;  // the last replacement kept the search direction
     double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
     // The host checks after every replay (of check_every iterations) whether
     // the FP32 residual has dropped by `drop`; unless the drop was foreseen,
@@ -1312,12 +1392,16 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     // residual is formed after that one.
     double pause_rr = ff * drop * drop;
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause_rr))) return st;
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause_rr, tail, tb, &pred)))
+        return st;
       const int done = ctx->h_sc->done;
       const double rr_rec = ctx->h_sc->rr_final;
       iters = ctx->h_sc->iter_final;
       femgpu::launch_flush_x(ctx->dm, x64, w.x, ctx->stream);
-      if (done != 0 && done != 1) {
+      // a breakdown right after a replacement that kept p: restart from the
+      // FP64 residual (no block of the failing update wrote x or r)
+      const bool retry = done == 3 && kept_p;
+      if (done != 0 && done != 1 && !retry) {
         status = done_status(done);
         break;
       }
@@ -1343,11 +1427,13 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
       // FP64 residual did not at least halve since the previous replacement:
       // then the directions belong to another residual.
       femgpu::launch_pcg_replace(ctx->dm, r64, w.r, w.z, w.invD, part_node, ctx->stream);
-      if (rel > 2.0 * rel_rec || rel > 0.5 * rel_prev)
-        femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
+      const bool restart = rel > 2.0 * rel_rec || rel > 0.5 * rel_prev || retry;
+      if (restart) femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
+      kept_p = !restart;
       if (done) femgpu::launch_pcg_resume(sc, ctx->stream);
       rel_prev = rel;
       pause_rr = ctx->h_sc->rr * drop * drop;
+      pred.seed(iters, ctx->h_sc->rr, ctx->h_sc->rr);
       ++replacements;
     }
     S.iterations = iters;

@@ -480,9 +480,12 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   const double gam = sc->g[a.parity];
   const double alpha_d = gam / pq;
   if (!(pq > 0.0) || !isfinite(alpha_d)) {
+    // no block writes x or r: the state is the last completed iteration's,
+    // as after a stop in the patch kernel's preamble at this parity
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->pq = pq;
       sc->done = isfinite(pq) ? 3 : 2;
+      sc->par_final = a.parity;
     }
     return;
   }

@@ -10,7 +10,7 @@ import pytest
 
 import cudafemproject_amd as F
 import oracle as O
-from cudafemproject_amd.problem import cube_problem, plate_problem
+from cudafemproject_amd.problem import cantilever_problem, cube_problem, plate_problem
 from helpers import dense_K, fixture_problem
 
 pytestmark = pytest.mark.gpu
@@ -143,3 +143,20 @@ def test_single_reduction_autograd_and_compliance(ctx):
     c2, g2, u2, _ = ctx.compliance_grad(F.FP64, tol=1e-12, **SINGLE)
     assert abs(c2 - c1) <= 1e-10 * abs(c1)
     assert rel(g2, g1) < 1e-9
+
+
+def test_single_reduction_mixed_restarts_after_breakdown(ctx):
+    """Cantilever 96x32x32 (589,824 tets, E_e = E0 rho_e^3): the first
+    replacement that keeps p is followed by a breakdown of the FP32
+    single-reduction recurrence; the solve restarts from the FP64 residual
+    instead of failing."""
+    ctx.set_problem(cantilever_problem())
+    u64, s64 = ctx.solve(F.FP64, tol=1e-8, tol_kind=F.TOL_REL_RES)
+    for rec in (F.CG_SINGLE_REDUCTION, F.CG_STANDARD):
+        um, sm = ctx.solve(F.MIXED, tol=1e-8, tol_kind=F.TOL_REL_RES, recurrence=rec)
+        print(f"[mixed cantilever] recurrence {rec}: it {sm['iterations']} (fp64 "
+              f"{s64['iterations']}), outer {sm['outer_iterations']}, rel_res {sm['rel_res']:.2e}",
+              flush=True)
+        assert sm["status"] == 0
+        assert sm["rel_res"] <= 1e-8 * 1.0001
+        assert rel(um, u64) < 1e-6
