@@ -182,3 +182,21 @@ def test_zero_rhs_gives_zero(ctx):
     ctx.set_problem(pb)
     u, st = ctx.solve(F.FP64, tol=1e-8)
     assert np.all(u == 0) and st["iterations"] == 0
+
+
+@pytest.mark.parametrize("rec", [F.CG_STANDARD, F.CG_SINGLE_REDUCTION])
+def test_replay_length_does_not_change_the_solve(ctx, rec):
+    """The number of iterations per graph replay (and the short replays run
+    near the stop) only changes when the host looks: FP64 and FP32 solves
+    give bitwise the same displacements and iteration counts."""
+    ctx.set_problem(cube_problem(10, convention=F.LOADS_REFERENCE))
+    for prec in (F.FP64, F.FP32):
+        ref = None
+        for ce in (2, 6, 32, 64):
+            u, s = ctx.solve(prec, tol=1e-9, tol_kind=F.TOL_REL_RES, check_every=ce, recurrence=rec)
+            assert s["status"] == 0
+            if ref is None:
+                ref = (u.copy(), s["iterations"])
+            else:
+                assert s["iterations"] == ref[1], (prec, ce)
+                assert np.array_equal(u, ref[0]), (prec, ce)
