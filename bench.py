@@ -193,6 +193,54 @@ def bench_column(args, F):
     ctx.close()
 
 
+def bench_implicit(args, F):
+    """Implicit GN22 (Newmark, beta1 = 0.5, beta2 = 0.5) on the 16x16x256
+    column, physical operator M + beta2 dt^2/2 K with a nested Jacobi-PCG per
+    step (eps 1e-8 on gamma), dt = 4x the explicit step, 200 steps."""
+    from cudafemproject_amd.problem import column_problem
+    pb = column_problem(16, 16, 256, 0.1, 0.1, 1.6, E=2e7, nu=0.25, pressure=1.0,
+                        convention=F.LOADS_PHYSICAL)
+    rho = 1000.0
+    ctx = F.Context(0)
+    ctx.set_problem(pb)
+    dt = 4.0 * 0.5 * 2.0 / omega_max(ctx, pb, lumped_mass(pb, rho))
+    nsteps = 200
+    res = {}
+    for name, prec in (("fp64", F.FP64), ("fp32", F.FP32)):
+        kw = dict(rho=rho, beta1=0.5, beta2=0.5, dt=dt, endtime=(nsteps + 0.5) * dt,
+                  wavelet_type=F.WAVELET_RICKER, freq=50.0, timeshift=0.0, ampl=1e3,
+                  precision=prec, mass_convention=F.MASS_PHYSICAL,
+                  implicit_mode=F.IMPLICIT_PHYSICAL, pcg_eps=1e-8)
+        for _ in range(args.warmup):
+            ctx.dyn_run(**dict(kw, endtime=20.5 * dt))
+        runs = [ctx.dyn_run(**kw) for _ in range(args.steps)]
+        u, st = runs[-1]
+        ms = np.mean([r[1]["run_ms"] for r in runs])
+        res[name] = (ms, st["steps"], st["pcg_iterations"], u)
+        log(f"implicit {name}: {st['steps']} steps, {st['pcg_iterations']} PCG iterations in "
+            f"{ms:.1f} ms = {ms * 1e3 / st['steps']:.1f} us/step")
+    ms64, n64, it64, u64 = res["fp64"]
+    ms32, n32, it32, u32 = res["fp32"]
+    line = {
+        "metric": "implicit GN22 step time (column 16x16x256, 200 steps, PCG per step)",
+        "value": ms64 * 1e3 / n64, "unit": "us/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "column 0.1x0.1x1.6, 16x16x256 hexes x 6 tets, bottom clamped, "
+                               "Ricker 50 Hz x 1e3 top pressure, rho=1000, Newmark beta1 0.5 "
+                               "beta2 0.5, operator M + beta2 dt^2/2 K, PCG eps 1e-8, "
+                               "dt = 4 x 0.5 * 2/omega_max", "n_elems": pb.n_elems,
+                   "n_dof": pb.n_dof, "dt": dt, "time_steps": n64},
+        "pcg_iterations_per_step_fp64": it64 / n64, "pcg_iterations_per_step_fp32": it32 / n32,
+        "us_per_step_fp32": ms32 * 1e3 / n32,
+        "us_per_pcg_iteration_fp64": ms64 * 1e3 / it64,
+        "fp32_vs_fp64_rel_displacement_diff": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def bench_cantilever(args, F):
     """Compliance and its adjoint gradient on the 96x32x32 cantilever; central
     FD (h = 1e-6 in rho_e) on 64 seeded elements.  The FD evaluates compliance
@@ -368,10 +416,11 @@ def traffic_probe(args, F):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["cube", "column", "cantilever", "batch"],
+    ap.add_argument("--workload", choices=["cube", "column", "implicit", "cantilever", "batch"],
                     default="cube",
                     help="cube: static solve (default, headline); column: 20,000-step explicit "
-                         "dynamics; cantilever: compliance + adjoint gradient; batch: 512 "
+                         "dynamics; implicit: 200 implicit Newmark steps on the same column; "
+                         "cantilever: compliance + adjoint gradient; batch: 512 "
                          "plates x 4 load cases in one launch")
     ap.add_argument("--dyn-steps", type=int, default=20000)
     ap.add_argument("--traffic-probe", action="store_true",
@@ -402,6 +451,8 @@ def main():
         return traffic_probe(args, F)
     if args.workload == "column":
         return bench_column(args, F)
+    if args.workload == "implicit":
+        return bench_implicit(args, F)
     if args.workload == "cantilever":
         return bench_cantilever(args, F)
     if args.workload == "batch":
