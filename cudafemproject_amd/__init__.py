@@ -8,10 +8,10 @@ from ._lib import (BATCH_LOCKSTEP, BATCH_PERSISTENT, CG_SINGLE_REDUCTION, CG_STA
                    LOADS_REFERENCE, MASS_PHYSICAL, MASS_REFERENCE, TOL_REF_GAMMA, TOL_REL_RES,
                    RECV_ACCELERATION, RECV_DISPLACEMENT, RECV_VELOCITY, WAVELET_BERLAGE,
                    WAVELET_NONE, WAVELET_RICKER, Context, FemGpuError,
-                   load_library)
+                   load_library, plan_info)
 from .problem import Problem
 
-__all__ = ["Context", "FemGpuError", "Problem", "load_library", "FP32", "FP64", "MIXED",
+__all__ = ["Context", "FemGpuError", "Problem", "load_library", "plan_info", "FP32", "FP64", "MIXED",
            "TOL_REF_GAMMA", "TOL_REL_RES", "LOADS_REFERENCE", "LOADS_PHYSICAL",
            "MASS_REFERENCE", "MASS_PHYSICAL", "WAVELET_NONE", "WAVELET_RICKER",
            "WAVELET_BERLAGE", "IMPLICIT_REFERENCE", "IMPLICIT_PHYSICAL", "CG_STANDARD",

@@ -79,6 +79,7 @@ SIGNATURES = {
     "femgpu_sync": (C.c_int, [_P]),
     "femgpu_set_mesh": (C.c_int, [_P, C.c_int, C.c_int64, _D, C.c_int64, _I]),
     "femgpu_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
+    "femgpu_plan_info": (C.c_int, [C.c_int, C.c_int64, _D, C.c_int64, _I, C.c_int, C.c_int, _L]),
     "femgpu_set_material": (C.c_int, [_P, C.c_double, C.c_double, _D]),
     "femgpu_set_constraints": (C.c_int, [_P, C.c_int64, _I, _I]),
     "femgpu_set_pressure_faces": (C.c_int, [_P, C.c_int64, _I, _I, _D, _D]),
@@ -131,6 +132,23 @@ def load_library(path: str | None = None):
     if path is None:
         _lib = lib
     return lib
+
+
+def plan_info(nodes: np.ndarray, elems: np.ndarray, elems_per_patch: int = 0,
+              patch_threads: int = 0) -> dict:
+    """The patch plan femgpu_set_mesh builds for a mesh at a tuning (host
+    only, no device needed)."""
+    lib = load_library()
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    info = np.zeros(6, np.int64)
+    st = lib.femgpu_plan_info(nodes.shape[1], nodes.shape[0], _dp(nodes), elems.shape[0],
+                              _ip(elems), elems_per_patch, patch_threads,
+                              info.ctypes.data_as(_L))
+    if st != OK:
+        raise FemGpuError(st, "femgpu_plan_info")
+    keys = ("patches", "owned", "interface", "slots", "elems_per_patch", "max_patch_nodes")
+    return dict(zip(keys, (int(v) for v in info)))
 
 
 class FemGpuError(RuntimeError):

@@ -58,19 +58,60 @@ def build_library(out: str = LIB) -> str:
     return out
 
 
-def build_fem_main() -> str | None:
-    """Compile the FEMproject-compatible driver executable (femproject/),
-    linked against libfemgpu.so with an rpath to this directory."""
+FEM_MAIN = os.path.join(PKG, "fem_main")
+FEM_MAIN_STAMP = FEM_MAIN + ".srchash"
+
+
+def _fem_main_sources() -> list[str]:
     src_dir = os.path.join(PKG, "femproject")
     if not os.path.isdir(src_dir):
-        return None
-    srcs = sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir) if f.endswith(".cpp"))
+        return []
+    return sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir)
+                  if f.endswith((".cpp", ".h")))
+
+
+def _fem_main_hash() -> str:
+    """sha256 over the driver's sources and the C-ABI header it compiles against."""
+    import hashlib
+    h = hashlib.sha256()
+    for p in _fem_main_sources() + [os.path.join(INCLUDE, "femgpu.h")]:
+        h.update(os.path.basename(p).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def fem_main_current() -> bool:
+    """fem_main exists and was built from the sources as they are now."""
+    if not (os.path.exists(FEM_MAIN) and os.path.exists(FEM_MAIN_STAMP)):
+        return False
+    with open(FEM_MAIN_STAMP) as f:
+        return f.read().strip() == _fem_main_hash()
+
+
+def build_fem_main() -> str | None:
+    """Compile the FEMproject-compatible driver executable (femproject/),
+    linked against libfemgpu.so with an rpath to this directory, and record
+    the hash of its sources beside it."""
+    src_dir = os.path.join(PKG, "femproject")
+    srcs = [p for p in _fem_main_sources() if p.endswith(".cpp")]
     if not srcs:
         return None
-    out = os.path.join(PKG, "fem_main")
-    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{src_dir}", "-o", out, *srcs,
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{src_dir}", "-o", FEM_MAIN, *srcs,
           f"-L{PKG}", "-lfemgpu", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
-    return out
+    with open(FEM_MAIN_STAMP, "w") as f:
+        f.write(_fem_main_hash() + "\n")
+    return FEM_MAIN
+
+
+def ensure_fem_main() -> str:
+    """fem_main as built from the current sources (rebuilt with g++ when it is
+    missing or stale; needs libfemgpu.so)."""
+    if not fem_main_current():
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run __graft_entry__.build()")
+        build_fem_main()
+    return FEM_MAIN
 
 
 def build_all():

@@ -149,6 +149,32 @@ int fail(femgpu_ctx *c, int code, const std::string &msg) {
       return fail(ctx, FEMGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// The device plan of a mesh: patches of <= emax_req elements (0: 3 x
+// threads) and <= threads nodes, one workgroup thread per patch node and at
+// most three elements per thread.  The FP64 patch kernels must fit a
+// workgroup's LDS (160 KiB on gfx950): patches with many nodes per element
+// (thin or irregular meshes) are re-planned with fewer elements.
+int plan_mesh(int dim, int64_t n_nodes, const double *nodes, int64_t n_elems, const int32_t *elems,
+              int emax_req, int threads, MeshPlan &plan, std::string &msg) {
+  int emax = emax_req > 0 ? emax_req : 3 * threads;
+  if (emax > 3 * threads) {
+    msg = "elements per patch above 3 x patch threads";
+    return FEMGPU_E_ARG;
+  }
+  constexpr int LDS_MAX = 160 * 1024;
+  for (;;) {
+    msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, threads, true, plan);
+    if (!msg.empty()) return FEMGPU_E_BAD_MESH;
+    const int ent = (plan.ent_used + 3) & ~3;
+    if (femgpu::lds_bytes(dim, 8, plan.pn_used, ent) <= LDS_MAX) return FEMGPU_OK;
+    if (emax <= 64) {
+      msg = "patch does not fit the LDS";
+      return FEMGPU_E_BAD_MESH;
+    }
+    emax = std::max(64, emax * 7 / 8);
+  }
+}
+
 void drop_graphs(femgpu_ctx *ctx) {
   for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
   ctx->graphs.clear();
@@ -793,20 +819,10 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   // iteration in FP64 (no register spills, three workgroups per CU) and
   // within a few percent of the fastest in FP32 among 256/512/1024.
   const int threads = ctx->threads_req ? ctx->threads_req : 256;
-  int emax = ctx->emax > 0 ? ctx->emax : 3 * threads;
-  if (emax > 3 * threads) return fail(ctx, FEMGPU_E_ARG, "elements per patch above 3 x patch threads");
-  // The FP64 patch kernels must fit a workgroup's LDS (160 KiB on gfx950):
-  // patches with many nodes per element (thin or irregular meshes) are
-  // re-planned with fewer elements.
-  constexpr int LDS_MAX = 160 * 1024;
-  std::string msg;
-  for (;;) {
-    msg = femgpu::build_plan(dim, n_nodes, nodes, n_elems, elems, emax, threads, true, ctx->plan);
-    if (!msg.empty()) return fail(ctx, FEMGPU_E_BAD_MESH, msg);
-    const int ent = (ctx->plan.ent_used + 3) & ~3;
-    if (femgpu::lds_bytes(dim, 8, ctx->plan.pn_used, ent) <= LDS_MAX) break;
-    if (emax <= 64) return fail(ctx, FEMGPU_E_BAD_MESH, "patch does not fit the LDS");
-    emax = std::max(64, emax * 7 / 8);
+  {
+    std::string msg;
+    const int st = plan_mesh(dim, n_nodes, nodes, n_elems, elems, ctx->emax, threads, ctx->plan, msg);
+    if (st) return fail(ctx, st, msg);
   }
   // degenerate elements
   for (int64_t e = 0; e < n_elems; ++e) {
@@ -930,6 +946,28 @@ int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads) {
   // takes effect at the next femgpu_set_mesh (the plan depends on both)
   ctx->emax = elems_per_patch;
   ctx->threads_req = patch_threads;
+  return FEMGPU_OK;
+}
+
+int femgpu_plan_info(int dim, int64_t n_nodes, const double *nodes, int64_t n_elems,
+                     const int32_t *elems, int elems_per_patch, int patch_threads, int64_t *info) {
+  if (!nodes || !elems || !info || (dim != 2 && dim != 3) || n_nodes <= 0 || n_elems <= 0)
+    return FEMGPU_E_ARG;
+  if (patch_threads != 0 && patch_threads != 256 && patch_threads != 512 && patch_threads != 1024)
+    return FEMGPU_E_ARG;
+  if (elems_per_patch != 0 && (elems_per_patch < 16 || elems_per_patch > femgpu::EMAX_LIMIT))
+    return FEMGPU_E_ARG;
+  MeshPlan plan;
+  std::string msg;
+  const int st = plan_mesh(dim, n_nodes, nodes, n_elems, elems, elems_per_patch,
+                           patch_threads ? patch_threads : 256, plan, msg);
+  if (st) return st;
+  info[0] = int64_t(plan.patches.size());
+  info[1] = plan.n_owned;
+  info[2] = plan.n_iface;
+  info[3] = plan.n_slots;
+  info[4] = plan.emax;
+  info[5] = plan.pn_used;
   return FEMGPU_OK;
 }
 

@@ -131,6 +131,14 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
  * No counterpart in the reference (its kernels use 256 threads and one
  * thread per element, fem_utils_kernels.cu:3). */
 int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads);
+/* The patch plan femgpu_set_mesh would build for this mesh and tuning,
+ * computed on the host (needs no device): info[0] patches, [1] nodes owned
+ * by one patch, [2] interface nodes, [3] interface partial-sum slots,
+ * [4] elements per patch planned with, [5] most nodes in one patch.
+ * No counterpart in the reference (it has no patches); for tests and
+ * tuning tools. */
+int femgpu_plan_info(int dim, int64_t n_nodes, const double *nodes, int64_t n_elems,
+                     const int32_t *elems, int elems_per_patch, int patch_threads, int64_t *info);
 /* Isotropic material (datakeeper.cpp:96-114, CreateMatrixD).  elem_scale:
  * optional per-element modulus factor (n_elems, user element order). */
 int femgpu_set_material(femgpu_ctx *ctx, double E, double nu, const double *elem_scale);

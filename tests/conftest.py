@@ -22,3 +22,11 @@ def ctx():
     c = F.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def fem_main():
+    """Path of the FEMproject-compatible driver, rebuilt (g++, seconds) when it
+    is missing or older than its sources."""
+    from cudafemproject_amd.build import ensure_fem_main
+    return ensure_fem_main()

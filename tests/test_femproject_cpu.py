@@ -10,7 +10,15 @@ import pytest
 
 from conftest import GOLDEN, ROOT
 
-FEM_MAIN = os.path.join(ROOT, "cudafemproject_amd", "fem_main")
+
+
+@pytest.fixture(autouse=True)
+def _driver(fem_main):
+    global FEM_MAIN
+    FEM_MAIN = fem_main
+
+
+FEM_MAIN = None
 
 
 def workspace(tmp_path, fixture="test_rect_pcg", dim=2, **cfg):

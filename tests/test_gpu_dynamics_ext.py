@@ -228,12 +228,12 @@ def test_relaxation_reference_stub_is_one_step(ctx):
     assert st["steps"] == 1
 
 
-def test_fem_main_writes_receivers_and_segy(tmp_path):
+def test_fem_main_writes_receivers_and_segy(tmp_path, fem_main):
     from test_femproject_cpu import workspace
     cfg = workspace(tmp_path, "test_rect_pcg", task_type="dynamic", rho=2400, dt=1e-5,
                     endtime=2.05e-3, beta1=0.5, beta2=0.0,
                     wavelet={"WaveletType": "ricker", "freq": 1000, "ampl": 1e3})
-    r = subprocess.run([os.path.join(ROOT, "cudafemproject_amd", "fem_main"), cfg, "test_rect_pcg",
+    r = subprocess.run([fem_main, cfg, "test_rect_pcg",
                         "--receivers", "1,3,5", "--receiver-axis", "0", "--segy-every", "4"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("fixture", ["test_rect_pcg", "9task_3"])
-def test_run_fem_static_fixture(tmp_path, fixture):
+def test_run_fem_static_fixture(tmp_path, fixture, fem_main):
     cfg = workspace(tmp_path, fixture, youngModulus=2e7, poissonRatio=0.25)
     r = subprocess.run(["python", os.path.join(ROOT, "run_fem.py"), cfg], capture_output=True,
                        text=True, timeout=120)
@@ -31,7 +31,7 @@ def test_run_fem_static_fixture(tmp_path, fixture):
     assert np.abs(got - sg).max() / np.abs(sg).max() < 1e-4
 
 
-def test_fem_main_argv_form_dynamic(tmp_path):
+def test_fem_main_argv_form_dynamic(tmp_path, fem_main):
     """run_fem.py:80-84 argv (task dim proj prep res nu E verbose 1 rho da db dt
     endtime beta1 beta2): explicit dynamics on the test_rect_pcg mesh vs the
     FP64 oracle."""
@@ -39,7 +39,7 @@ def test_fem_main_argv_form_dynamic(tmp_path):
     prep = tmp_path / "prepared_meshes" / "2D" / "test_rect_pcg"
     res = tmp_path / "res"
     res.mkdir()
-    exe = os.path.join(ROOT, "cudafemproject_amd", "fem_main")
+    exe = fem_main
     r = subprocess.run([exe, "test_rect_pcg", "2", ROOT, str(prep), str(res), "0.25", "2e7", "0",
                         "1", "2400", "0", "0", "1e-6", "2e-4", "0.5", "0"],
                        capture_output=True, text=True, timeout=120)
@@ -51,7 +51,7 @@ def test_fem_main_argv_form_dynamic(tmp_path):
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-7   # 9 printed digits
 
 
-def test_run_fem_from_raw_k_mesh(tmp_path):
+def test_run_fem_from_raw_k_mesh(tmp_path, fem_main):
     """The full reference flow from the raw LS-DYNA mesh: run_fem.py parses
     raw_meshes/2D/9task_3.k, solves, and results.vtk matches the golden
     results the reference's tests hold (run_tests.py:42,67-69 criterion)."""

@@ -202,6 +202,50 @@ def test_static_fp64_matches_dense_direct_solve(gen):
     assert np.abs(u.ravel()[fx]).max() <= 1e-12 * np.abs(ud).max()
 
 
+@pytest.mark.parametrize("gen", ["plate", "cube", "cube_scaled"])
+def test_diag_equals_dense_operator_diagonal(gen):
+    """orc_diag (getDiagonal + reductionWithMask, cpu_matrix.cpp:296-305) is the
+    diagonal of the assembled constrained operator the oracle's matvec
+    applies (each column of K is a product with a unit vector), entry by
+    entry, constrained rows included."""
+    if gen == "plate":
+        pb = P.plate_problem(7, 3)
+    else:
+        pb = P.cube_problem(3, convention=0)
+        if gen == "cube_scaled":
+            pb.elem_scale = np.random.default_rng(2).uniform(0.1, 1.0, pb.n_elems)
+    d = O.diag(pb, np.float64).ravel()
+    kd = np.diag(dense_K(pb))
+    assert np.all(kd > 0)
+    np.testing.assert_allclose(d, kd, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_stress_of_linear_field(dim):
+    """Per-element stress of u = G x is D eps(G) on every element (Voigt
+    xx, yy, [zz,] xy, [yz, xz], engineering shears), times the element's
+    modulus scale, whatever the element's orientation: a third of the
+    elements have two corners swapped (negative determinant)."""
+    rng = np.random.default_rng(11)
+    pb = P.plate_problem(6, 4, E=3e7, nu=0.3) if dim == 2 else \
+        P.cube_problem(3, E=3e7, nu=0.3, convention=0)
+    el = pb.elems.copy()
+    flip = np.arange(pb.n_elems) % 3 == 1
+    el[flip, -2], el[flip, -1] = pb.elems[flip, -1], pb.elems[flip, -2]
+    pb.elems = el
+    pb.elem_scale = rng.uniform(0.2, 1.0, pb.n_elems)
+    G = rng.standard_normal((dim, dim))
+    u = pb.nodes @ G.T
+    s = O.stress(pb, u, np.float64)
+    if dim == 2:
+        eps = np.array([G[0, 0], G[1, 1], G[0, 1] + G[1, 0]])
+    else:
+        eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1],
+                        G[0, 2] + G[2, 0]])
+    want = pb.elem_scale[:, None] * (_D(dim, pb.E, pb.nu) @ eps)[None, :]
+    np.testing.assert_allclose(s, want, rtol=0, atol=1e-9 * np.abs(want).max())
+
+
 def test_rhs_reference_convention_2d():
     """2D pressure: F = -1/2 p L n per edge node, assigned once per element
     slot (the duplicated segment of test_rect_pcg counts once)."""
