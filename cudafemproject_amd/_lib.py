@@ -79,6 +79,8 @@ SIGNATURES = {
     "femgpu_sync": (C.c_int, [_P]),
     "femgpu_set_mesh": (C.c_int, [_P, C.c_int, C.c_int64, _D, C.c_int64, _I]),
     "femgpu_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
+    "femgpu_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]),
     "femgpu_plan_info": (C.c_int, [C.c_int, C.c_int64, _D, C.c_int64, _I, C.c_int, C.c_int, _L]),
     "femgpu_set_material": (C.c_int, [_P, C.c_double, C.c_double, _D]),
     "femgpu_set_constraints": (C.c_int, [_P, C.c_int64, _I, _I]),
@@ -94,6 +96,7 @@ SIGNATURES = {
     "femgpu_default_solve_opts": (C.c_int, [C.POINTER(SolveOpts)]),
     "femgpu_solve_static": (C.c_int, [_P, C.POINTER(SolveOpts), _D, C.POINTER(Stats)]),
     "femgpu_stress": (C.c_int, [_P, _D, _D]),
+    "femgpu_strain": (C.c_int, [_P, _D, _D]),
     "femgpu_element_energy": (C.c_int, [_P, _D, _D]),
     "femgpu_batch_set_mesh": (C.c_int, [_P, C.c_int, C.c_int, _L, _D, _L, _I]),
     "femgpu_batch_set_material": (C.c_int, [_P, _D, _D]),
@@ -212,6 +215,15 @@ class Context:
 
     def set_tuning(self, elems_per_patch: int = 0, patch_threads: int = 0):
         self._check(self.lib.femgpu_set_tuning(self.h, elems_per_patch, patch_threads), "set_tuning")
+
+    def autotune(self, precision: int = MIXED) -> dict:
+        """femgpu_autotune: pick and apply the fastest patch tuning for the
+        current problem (cached per mesh)."""
+        e, t, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._check(self.lib.femgpu_autotune(self.h, precision, C.byref(e), C.byref(t),
+                                             C.byref(ms)), "autotune")
+        return {"elems_per_patch": e.value, "patch_threads": t.value,
+                "ms_per_iteration": ms.value}
 
     def set_material(self, E: float, nu: float, elem_scale: np.ndarray | None = None):
         if elem_scale is not None:
@@ -434,6 +446,14 @@ class Context:
         nv = 3 * (self.dim - 1)
         s = np.zeros((self.n_elems, nv))
         self._check(self.lib.femgpu_stress(self.h, _dp(u), _dp(s)), "stress")
+        return s
+
+    def strain(self, u: np.ndarray) -> np.ndarray:
+        """Per-element engineering strain B u_e (n_elems, 3(dim-1))."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nv = 3 * (self.dim - 1)
+        s = np.zeros((self.n_elems, nv))
+        self._check(self.lib.femgpu_strain(self.h, _dp(u), _dp(s)), "strain")
         return s
 
     def element_energy(self, u: np.ndarray) -> np.ndarray:

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -116,6 +118,7 @@ struct femgpu_ctx {
   CgScalars *h_sc = nullptr;  // pinned, h_sc[0..3]: poll ring
   hipEvent_t ev_poll[4] = {nullptr, nullptr, nullptr, nullptr};
   int emax_req = 0, threads_req = 0;
+  int tuned_threads = 256;  // threads per patch of the current plan
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   // graph cache for PCG iteration blocks, keyed by everything baked into
@@ -738,6 +741,94 @@ int true_rel_res(femgpu_ctx *ctx, const double *f, const double *x, double *out)
 
 }  // namespace
 
+
+static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char *f_name,
+                      double *u_out, femgpu_stats *stats);
+static bool trace_enabled();
+
+namespace {
+
+// The device mesh of ctx->plan (planned with `ctx->tuned_threads` threads per
+// patch): patch records, node runs, interface slots, coordinates; then the
+// operator's per-node data is rebuilt lazily from the user-order problem
+// (material, constraint mask, diagonal, loads).
+int upload_plan(femgpu_ctx *ctx) {
+  const int threads = ctx->tuned_threads;
+  const int dim = ctx->dim;
+  const int64_t n_nodes = ctx->nn;
+  const double *nodes = ctx->nodes.data();
+  const MeshPlan &P = ctx->plan;
+  DevMesh &dm = ctx->dm;
+  dm = DevMesh();
+  dm.dim = dim;
+  dm.np = int(P.patches.size());
+  dm.emax = P.emax_used;
+  dm.ent = (P.ent_used + 3) & ~3;
+  dm.pn = P.pn_used;
+  dm.nn = P.nn;
+  dm.ne = P.ne;
+  dm.n_owned = P.n_owned;
+  dm.n_iface = P.n_iface;
+  dm.n_active = P.n_owned + P.n_iface;
+  dm.patch_threads = threads;
+  if (dm.pn > threads || dm.emax > 3 * threads)
+    return fail(ctx, FEMGPU_E_ARG, "internal error: patch larger than the workgroup");
+  int st;
+  femgpu::PatchInfo *pp;
+  uint32_t *rec;
+  int32_t *shg, *shs, *ss, *ei;
+  uint16_t *ne_;
+  std::vector<uint32_t> recs;
+  femgpu::pack_records(P, recs);
+  if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "erec", recs, &rec)) ||
+      (st = upload(ctx, "sh_gid", P.sh_gid, &shg)) || (st = upload(ctx, "sh_slot", P.sh_slot, &shs)) ||
+      (st = upload(ctx, "node_end", P.node_end, &ne_)) ||
+      (st = upload(ctx, "slot_start", P.slot_start, &ss)) ||
+      (st = upload(ctx, "elems_int", P.elems_int, &ei)))
+    return st;
+
+  dm.patches = pp;
+  dm.erec = rec;
+  dm.sh_gid = shg;
+  dm.sh_slot = shs;
+  {
+    // per patch interface node: id, own slot, the node's slot range
+    std::vector<int32_t> info(P.sh_gid.size() * 4);
+    for (size_t i = 0; i < P.sh_gid.size(); ++i) {
+      const int64_t s = P.sh_gid[i] - P.n_owned;
+      info[4 * i + 0] = P.sh_gid[i];
+      info[4 * i + 1] = P.sh_slot[i];
+      info[4 * i + 2] = P.slot_start[s];
+      info[4 * i + 3] = P.slot_start[s + 1];
+    }
+    int32_t *pi;
+    if ((st = upload(ctx, "sh_info", info, &pi))) return st;
+    dm.sh_info = reinterpret_cast<const int4 *>(pi);
+  }
+  dm.node_end = ne_;
+  dm.slot_start = ss;
+  dm.elems_int = ei;
+  std::vector<double> c64(size_t(n_nodes) * dim);
+  std::vector<float> c32(c64.size());
+  for (int64_t i = 0; i < n_nodes; ++i)
+    for (int c = 0; c < dim; ++c) {
+      c64[size_t(i) * dim + c] = nodes[size_t(P.node_perm[i]) * dim + c];
+      c32[size_t(i) * dim + c] = float(c64[size_t(i) * dim + c]);
+    }
+  double *pc64;
+  float *pc32;
+  if ((st = upload(ctx, "coords64", c64, &pc64)) || (st = upload(ctx, "coords32", c32, &pc32)))
+    return st;
+  dm.coords64 = pc64;
+  dm.coords32 = pc32;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->have_mesh = true;
+  ctx->dirty_mat = ctx->dirty_fix = ctx->dirty_diag = ctx->dirty_rhs = true;
+  return ensure_fix(ctx);
+}
+
+}  // namespace
+
 extern "C" {
 
 int femgpu_create(femgpu_ctx **out, int device_ordinal) {
@@ -865,73 +956,8 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
   ctx->fix_user.assign(n_nodes, 0);
   ctx->ctype.assign(n_nodes, 0);
 
-  const MeshPlan &P = ctx->plan;
-  DevMesh &dm = ctx->dm;
-  dm = DevMesh();
-  dm.dim = dim;
-  dm.np = int(P.patches.size());
-  dm.emax = P.emax_used;
-  dm.ent = (P.ent_used + 3) & ~3;
-  dm.pn = P.pn_used;
-  dm.nn = P.nn;
-  dm.ne = P.ne;
-  dm.n_owned = P.n_owned;
-  dm.n_iface = P.n_iface;
-  dm.n_active = P.n_owned + P.n_iface;
-  dm.patch_threads = threads;
-  if (dm.pn > threads || dm.emax > 3 * threads)
-    return fail(ctx, FEMGPU_E_ARG, "internal error: patch larger than the workgroup");
-  int st;
-  femgpu::PatchInfo *pp;
-  uint32_t *rec;
-  int32_t *shg, *shs, *ss, *ei;
-  uint16_t *ne_;
-  std::vector<uint32_t> recs;
-  femgpu::pack_records(P, recs);
-  if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "erec", recs, &rec)) ||
-      (st = upload(ctx, "sh_gid", P.sh_gid, &shg)) || (st = upload(ctx, "sh_slot", P.sh_slot, &shs)) ||
-      (st = upload(ctx, "node_end", P.node_end, &ne_)) ||
-      (st = upload(ctx, "slot_start", P.slot_start, &ss)) ||
-      (st = upload(ctx, "elems_int", P.elems_int, &ei)))
-    return st;
-  dm.patches = pp;
-  dm.erec = rec;
-  dm.sh_gid = shg;
-  dm.sh_slot = shs;
-  {
-    // per patch interface node: id, own slot, the node's slot range
-    std::vector<int32_t> info(P.sh_gid.size() * 4);
-    for (size_t i = 0; i < P.sh_gid.size(); ++i) {
-      const int64_t s = P.sh_gid[i] - P.n_owned;
-      info[4 * i + 0] = P.sh_gid[i];
-      info[4 * i + 1] = P.sh_slot[i];
-      info[4 * i + 2] = P.slot_start[s];
-      info[4 * i + 3] = P.slot_start[s + 1];
-    }
-    int32_t *pi;
-    if ((st = upload(ctx, "sh_info", info, &pi))) return st;
-    dm.sh_info = reinterpret_cast<const int4 *>(pi);
-  }
-  dm.node_end = ne_;
-  dm.slot_start = ss;
-  dm.elems_int = ei;
-  std::vector<double> c64(size_t(n_nodes) * dim);
-  std::vector<float> c32(c64.size());
-  for (int64_t i = 0; i < n_nodes; ++i)
-    for (int c = 0; c < dim; ++c) {
-      c64[size_t(i) * dim + c] = nodes[size_t(P.node_perm[i]) * dim + c];
-      c32[size_t(i) * dim + c] = float(c64[size_t(i) * dim + c]);
-    }
-  double *pc64;
-  float *pc32;
-  if ((st = upload(ctx, "coords64", c64, &pc64)) || (st = upload(ctx, "coords32", c32, &pc32)))
-    return st;
-  dm.coords64 = pc64;
-  dm.coords32 = pc32;
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  ctx->have_mesh = true;
-  ctx->dirty_mat = ctx->dirty_fix = ctx->dirty_diag = ctx->dirty_rhs = true;
-  return ensure_fix(ctx);
+  ctx->tuned_threads = threads;
+  return upload_plan(ctx);
 }
 
 int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads) {
@@ -968,6 +994,146 @@ int femgpu_plan_info(int dim, int64_t n_nodes, const double *nodes, int64_t n_el
   info[3] = plan.n_slots;
   info[4] = plan.emax;
   info[5] = plan.pn_used;
+  return FEMGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Autotuner: candidate patch tunings (threads per patch, elements per patch),
+// the process-wide cache of choices per (mesh signature, arithmetic), and
+// the mesh signature (FNV-1a over dim, sizes, coordinates and connectivity).
+constexpr int TUNE_CAND[][2] = {{256, 768}, {256, 640}, {256, 512}, {512, 1536},
+                                {512, 1024}, {1024, 3072}, {1024, 2048}};
+constexpr int TUNE_N = int(sizeof(TUNE_CAND) / sizeof(TUNE_CAND[0]));
+
+struct TuneChoice {
+  int threads, elems;
+  double ms;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<uint64_t, int>, TuneChoice> g_tune_cache;
+
+uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+  const unsigned char *b = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+uint64_t mesh_signature(const femgpu_ctx *ctx) {
+  uint64_t h = 1469598103934665603ULL;
+  const int64_t head[3] = {ctx->dim, ctx->nn, ctx->ne};
+  h = fnv1a(h, head, sizeof(head));
+  h = fnv1a(h, ctx->nodes.data(), ctx->nodes.size() * sizeof(double));
+  return fnv1a(h, ctx->elems.data(), ctx->elems.size() * sizeof(int32_t));
+}
+
+// Re-plan the current mesh with a given tuning, keeping the problem.
+int replan(femgpu_ctx *ctx, MeshPlan &&plan, int threads) {
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  ctx->plan = std::move(plan);
+  ctx->tuned_threads = threads;
+  ctx->have_mesh = false;
+  int st = upload_plan(ctx);
+  if (st) return st;
+  ctx->have_mesh = true;
+  return FEMGPU_OK;
+}
+
+// Device time per PCG iteration of the current plan: the second of two
+// runs of `iters` iterations (the first captures the iteration graphs) on a
+// unit right-hand side, stopping rule out of reach.
+int time_pcg_iteration(femgpu_ctx *ctx, int precision, int iters, double *ms) {
+  const size_t nd = size_t(ctx->nn) * ctx->dim;
+  double *f;
+  int st;
+  if ((st = dbuf(ctx, "tune_f", nd, &f))) return st;
+  femgpu::launch_fill<double>(f, int64_t(nd), 1.0, ctx->stream);
+  femgpu_solve_opts o;
+  femgpu_default_solve_opts(&o);
+  o.precision = precision == FEMGPU_FP64 ? FEMGPU_FP64 : FEMGPU_FP32;
+  o.tol_kind = FEMGPU_TOL_REL_RES;
+  o.tol = 0.0;
+  o.max_iter = iters;
+  femgpu_stats S{};
+  for (int rep = 0; rep < 2; ++rep) {
+    st = solve_core(ctx, &o, "tune_f", nullptr, &S);
+    if (st != FEMGPU_OK && st != FEMGPU_E_MAXIT) return st;
+  }
+  if (S.iterations < iters / 2) return fail(ctx, FEMGPU_E_BREAKDOWN, "autotune: PCG stopped early");
+  *ms = S.solve_ms / S.iterations;
+  return FEMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *patch_threads,
+                    double *ms_per_iteration) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (precision != FEMGPU_FP64 && precision != FEMGPU_FP32 && precision != FEMGPU_MIXED)
+    return fail(ctx, FEMGPU_E_ARG, "unknown precision");
+  const int arith = precision == FEMGPU_FP64 ? FEMGPU_FP64 : FEMGPU_FP32;
+  const auto key = std::make_pair(mesh_signature(ctx), arith);
+  TuneChoice best{0, 0, 0.0};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      best = it->second;
+      cached = true;
+    }
+  }
+  if (!cached) {
+    // plan every candidate on the host in parallel, then time each on the GPU
+    std::vector<MeshPlan> plans(TUNE_N);
+    std::vector<int> pst(TUNE_N, FEMGPU_OK);
+    {
+      std::vector<std::thread> th;
+      for (int i = 0; i < TUNE_N; ++i)
+        th.emplace_back([&, i] {
+          std::string msg;
+          pst[i] = plan_mesh(ctx->dim, ctx->nn, ctx->nodes.data(), ctx->ne, ctx->elems.data(),
+                             TUNE_CAND[i][1], TUNE_CAND[i][0], plans[i], msg);
+        });
+      for (auto &t : th) t.join();
+    }
+    for (int i = 0; i < TUNE_N; ++i) {
+      if (pst[i] != FEMGPU_OK) continue;
+      if ((st = replan(ctx, std::move(plans[i]), TUNE_CAND[i][0]))) return st;
+      double ms = 0.0;
+      st = time_pcg_iteration(ctx, arith, 64, &ms);
+      if (st == FEMGPU_E_BREAKDOWN) continue;
+      if (st) return st;
+      if (trace_enabled())
+        std::fprintf(stderr, "femgpu autotune: %d threads, %d elements per patch: %.4f ms/iteration\n",
+                     TUNE_CAND[i][0], TUNE_CAND[i][1], ms);
+      if (best.threads == 0 || ms < best.ms) best = {TUNE_CAND[i][0], TUNE_CAND[i][1], ms};
+    }
+    if (best.threads == 0) return fail(ctx, FEMGPU_E_STATE, "autotune: no candidate ran");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  // leave the context planned with the choice, and keep it as its tuning
+  ctx->emax = best.elems;
+  ctx->threads_req = best.threads;
+  MeshPlan plan;
+  std::string msg;
+  if ((st = plan_mesh(ctx->dim, ctx->nn, ctx->nodes.data(), ctx->ne, ctx->elems.data(), best.elems,
+                      best.threads, plan, msg)))
+    return fail(ctx, st, msg);
+  if ((st = replan(ctx, std::move(plan), best.threads))) return st;
+  if (elems_per_patch) *elems_per_patch = best.elems;
+  if (patch_threads) *patch_threads = best.threads;
+  if (ms_per_iteration) *ms_per_iteration = best.ms;
   return FEMGPU_OK;
 }
 
@@ -1104,6 +1270,7 @@ int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y) {
   from_internal(ctx, yi, y);
   return FEMGPU_OK;
 }
+
 
 int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps) {
   int st = need_mesh(ctx);
@@ -1585,22 +1752,33 @@ int femgpu_compliance_grad(femgpu_ctx *ctx, const femgpu_solve_opts *opts, doubl
   return FEMGPU_OK;
 }
 
-int femgpu_stress(femgpu_ctx *ctx, const double *u, double *sigma_out) {
+namespace {
+// Per-element stress (strain = false) or engineering strain (true) of u.
+int element_tensor(femgpu_ctx *ctx, const double *u, double *out, bool strain) {
   int st = need_mesh(ctx);
   if (st) return st;
-  if (!u || !sigma_out) return FEMGPU_E_ARG;
+  if (!u || !out) return FEMGPU_E_ARG;
   if ((st = ensure_material(ctx))) return st;
   const int nv = 3 * (ctx->dim - 1);
   std::vector<double> ui = to_internal(ctx, u), s(size_t(ctx->ne) * nv);
   double *du, *ds;
   if ((st = upload(ctx, "st_u", ui, &du)) || (st = dbuf(ctx, "st_s", s.size(), &ds))) return st;
-  femgpu::launch_stress<double>(ctx->dm, du, ds, ctx->stream);
+  femgpu::launch_stress<double>(ctx->dm, du, ds, ctx->stream, strain);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(s.data(), ds, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   for (int64_t i = 0; i < ctx->ne; ++i)
-    for (int k = 0; k < nv; ++k) sigma_out[size_t(ctx->plan.elem_perm[i]) * nv + k] = s[size_t(i) * nv + k];
+    for (int k = 0; k < nv; ++k) out[size_t(ctx->plan.elem_perm[i]) * nv + k] = s[size_t(i) * nv + k];
   return FEMGPU_OK;
+}
+}  // namespace
+
+int femgpu_stress(femgpu_ctx *ctx, const double *u, double *sigma_out) {
+  return element_tensor(ctx, u, sigma_out, false);
+}
+
+int femgpu_strain(femgpu_ctx *ctx, const double *u, double *eps_out) {
+  return element_tensor(ctx, u, eps_out, true);
 }
 
 int femgpu_element_energy(femgpu_ctx *ctx, const double *u, double *e_out) {

@@ -129,7 +129,7 @@ void launch_convert(const TA *a, TB *b, int64_t n, hipStream_t s);
 
 // Per-element outputs in internal element order.
 template <typename T>
-void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s);
+void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s, bool strain = false);
 template <typename T>
 void launch_energy(const DevMesh &dm, const T *u, T *en, hipStream_t s);
 template <typename T>

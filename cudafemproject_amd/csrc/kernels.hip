@@ -709,7 +709,8 @@ __device__ __forceinline__ void load_elem(const int32_t *elems, const T *coords,
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(NT) void k_stress(int64_t ne, const int32_t *elems, const T *coords,
-                                               const T *escale, T lam, T mu, const T *u, T *sigma) {
+                                               const T *escale, T lam, T mu, const T *u, T *sigma,
+                                               bool strain) {
   constexpr int NDE = DIM * (DIM + 1), NV = 3 * (DIM - 1);
   const int64_t e = int64_t(blockIdx.x) * NT + threadIdx.x;
   if (e >= ne) return;
@@ -741,9 +742,19 @@ __global__ __launch_bounds__(NT) void k_stress(int64_t ne, const int32_t *elems,
 #pragma unroll
         for (int j = 0; j < 3; ++j) H[i][j] += du * c[a][j];
       }
+    T *o = sigma + e * NV;
+    if (strain) {  // engineering strain B u_e: xx, yy, zz, 2 xy, 2 yz, 2 xz
+      const T k = T(1) / det;
+      o[0] = k * H[0][0];
+      o[1] = k * H[1][1];
+      o[2] = k * H[2][2];
+      o[3] = k * (H[0][1] + H[1][0]);
+      o[4] = k * (H[1][2] + H[2][1]);
+      o[5] = k * (H[0][2] + H[2][0]);
+      return;
+    }
     const T k = es / det;
     const T tr = H[0][0] + H[1][1] + H[2][2];
-    T *o = sigma + e * NV;
     o[0] = k * (lam * tr + T(2) * mu * H[0][0]);
     o[1] = k * (lam * tr + T(2) * mu * H[1][1]);
     o[2] = k * (lam * tr + T(2) * mu * H[2][2]);
@@ -768,9 +779,16 @@ __global__ __launch_bounds__(NT) void k_stress(int64_t ne, const int32_t *elems,
       H10 += dy * c[a][0];
       H11 += dy * c[a][1];
     }
+    T *o = sigma + e * NV;
+    if (strain) {  // engineering strain B u_e: xx, yy, 2 xy
+      const T k = T(1) / det;
+      o[0] = k * H00;
+      o[1] = k * H11;
+      o[2] = k * (H01 + H10);
+      return;
+    }
     const T k = es / det;
     const T tr = H00 + H11;
-    T *o = sigma + e * NV;
     o[0] = k * (lam * tr + T(2) * mu * H00);
     o[1] = k * (lam * tr + T(2) * mu * H11);
     o[2] = k * mu * (H01 + H10);
@@ -1198,7 +1216,7 @@ void launch_convert(const TA *a, TB *b, int64_t n, hipStream_t s) {
 }
 
 template <typename T>
-void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s) {
+void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s, bool strain) {
   const T *coords = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.coords64)
                                    : reinterpret_cast<const T *>(dm.coords32);
   const T *es = sizeof(T) == 8 ? reinterpret_cast<const T *>(dm.escale64)
@@ -1206,10 +1224,10 @@ void launch_stress(const DevMesh &dm, const T *u, T *sigma, hipStream_t s) {
   const int g = int((dm.ne + NT - 1) / NT);
   if (dm.dim == 3)
     hipLaunchKernelGGL((k_stress<T, 3>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords, es,
-                       T(dm.lam), T(dm.mu), u, sigma);
+                       T(dm.lam), T(dm.mu), u, sigma, strain);
   else
     hipLaunchKernelGGL((k_stress<T, 2>), dim3(g), dim3(NT), 0, s, dm.ne, dm.elems_int, coords, es,
-                       T(dm.lam), T(dm.mu), u, sigma);
+                       T(dm.lam), T(dm.mu), u, sigma, strain);
 }
 
 template <typename T>
@@ -1363,7 +1381,7 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
                                    hipStream_t);                                               \
   template void launch_fill<T>(T *, int64_t, T, hipStream_t);                                  \
   template void launch_copy<T>(const T *, T *, int64_t, hipStream_t);                          \
-  template void launch_stress<T>(const DevMesh &, const T *, T *, hipStream_t);                \
+  template void launch_stress<T>(const DevMesh &, const T *, T *, hipStream_t, bool);                \
   template void launch_energy<T>(const DevMesh &, const T *, T *, hipStream_t);                \
   template void launch_bilinear<T>(const DevMesh &, const T *, const T *, double, T *, hipStream_t); \
   template void launch_dyn_predict<T>(const DevMesh &, T *, T *, T *, DynCoeffs, bool,         \

@@ -153,7 +153,7 @@ void dataKeeper::parseFiles() {
       facePress_.push_back(v[2 * dim_ + off + 1]);
     }
   }
-  displ_.assign(size_t(nn) * dim_, 0.0);
+  displ_.resize(size_t(nn), size_t(dim_));
 }
 
 void dataKeeper::setOutput(const std::string &resDir) {

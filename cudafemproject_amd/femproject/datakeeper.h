@@ -18,6 +18,35 @@
 
 namespace femproject {
 
+// Host matrix of doubles, row-major (rows x cols): the role of the
+// reference's Matrix for the results the solver hands back
+// (matrix_pack/include/matrix_pack.h; dk.get_displacements() is n x dim).
+class Matrix {
+ public:
+  Matrix() = default;
+  Matrix(size_t rows, size_t cols) : rows_(rows), cols_(cols), v_(rows * cols, 0.0) {}
+  void resize(size_t rows, size_t cols) {
+    rows_ = rows;
+    cols_ = cols;
+    v_.assign(rows * cols, 0.0);
+  }
+  size_t get_numRows() const { return rows_; }
+  size_t get_numCols() const { return cols_; }
+  size_t get_numElements() const { return v_.size(); }
+  double &operator()(size_t i, size_t j) { return v_[i * cols_ + j]; }
+  double operator()(size_t i, size_t j) const { return v_[i * cols_ + j]; }
+  double &operator[](size_t k) { return v_[k]; }
+  double operator[](size_t k) const { return v_[k]; }
+  double *get_data() { return v_.data(); }
+  const double *get_data() const { return v_.data(); }
+  std::vector<double> &values() { return v_; }
+  const std::vector<double> &values() const { return v_; }
+
+ private:
+  size_t rows_ = 0, cols_ = 0;
+  std::vector<double> v_;
+};
+
 struct MechanicalParams {  // datakeeper.h:21-30
   double poissonRatio = 0.3, youngModulus = 2e7, rho = 2400;
   double damping_alpha = 0, damping_beta = 0;
@@ -72,9 +101,15 @@ class dataKeeper {
   }
   void setOutput(const std::string &resDir);
 
-  std::vector<double> &displacements() { return displ_; }             // n x dim
-  const std::vector<double> &displacements() const { return displ_; }
-  std::vector<double> &stresses() { return stress_; }                 // m x 3(dim-1), optional
+  std::vector<double> &displacements() { return displ_.values(); }    // n x dim
+  const std::vector<double> &displacements() const { return displ_.values(); }
+  Matrix *get_displacements() { return &displ_; }                     // datakeeper.h:116
+  // per-element results of the last static solve (Voigt, 3(dim-1) each),
+  // empty when they were not computed
+  std::vector<double> &stresses() { return stress_; }                 // sigma = D B u_e
+  const std::vector<double> &stresses() const { return stress_; }
+  std::vector<double> &strains() { return strain_; }                  // B u_e (engineering)
+  const std::vector<double> &strains() const { return strain_; }
 
   void ShowInfo() const;
 
@@ -96,7 +131,34 @@ class dataKeeper {
   std::vector<double> faceNormals_, facePress_;
   std::vector<int32_t> loadNodes_;
   std::vector<double> loadVecs_;
-  std::vector<double> displ_, stress_;
+  Matrix displ_;
+  std::vector<double> stress_, strain_;
+};
+
+// Per-element and nodal post-processing results written to results.vtk
+// (DataKeeper/src/datakeeper.h:200-226, same constructor).  Stress and
+// Deformation hold 3(dim-1) Voigt values per element (Deformation: B u_e,
+// engineering shears), sigma_mises / epsilon_mises one value per element,
+// SmoothStress one value per node: the L2 projection of sigma_xx onto the
+// linear nodal basis when withSmooth, else zeros (the reference's main
+// passes withSmooth = false, and its fixtures hold zeros).
+// withStressAlongAxis (the reference's line-sampling text files) is kept
+// for the signature and writes nothing.
+class ResultsDataKeeper {
+ public:
+  ResultsDataKeeper(bool withStressAlongAxis, bool withSmooth, bool withMises, int64_t nodesCount)
+      : withStressAlongAxis(withStressAlongAxis), withSmooth(withSmooth), withMises(withMises),
+        SmoothStress(size_t(nodesCount > 0 ? nodesCount : 0), 0.0) {}
+
+  bool withStressAlongAxis;
+  bool withSmooth;
+  bool withMises;
+
+  std::vector<double> Deformation;
+  std::vector<double> Stress;
+  std::vector<double> sigma_mises;
+  std::vector<double> epsilon_mises;
+  std::vector<double> SmoothStress;
 };
 
 }  // namespace femproject

@@ -35,6 +35,42 @@ void require_hip(DEVICE_NAME d) {
 
 // ElementsData::setElementsData + the loads of initFlocals (fem_utils.cpp:41-51,
 // cpu_fem_utils.cpp:522-530): the whole problem goes to the device once.
+void load_problem(const Ctx &c, const dataKeeper &dk, const SolverOptions &opt);
+
+}  // namespace
+
+ElementsData *ElementsData::setElementsData(DEVICE_NAME device, const dataKeeper &dk,
+                                            const SolverOptions &opt) {
+  require_hip(device);
+  std::unique_ptr<ElementsData> ed(new ElementsData());
+  ed->opt_ = opt;
+  Ctx c(opt.device);
+  load_problem(c, dk, opt);
+  ed->h_ = c.h;
+  c.h = nullptr;  // owned by ed from here
+  ed->nn_ = dk.get_nodesCount();
+  ed->dim_ = dk.get_dim();
+  return ed.release();
+}
+
+ElementsData::~ElementsData() { femgpu_destroy(h_); }
+
+void ElementsData::calculateKlocals() {
+  std::vector<double> d(size_t(nn_) * dim_);
+  const int st = femgpu_diag(h_, d.data());
+  if (st != FEMGPU_OK)
+    throw std::runtime_error(std::string("calculateKlocals: ") + femgpu_last_error(h_));
+}
+
+void ElementsData::initFlocals(float /*t*/, const WaveletParams & /*wavelet*/) {
+  std::vector<double> f(size_t(nn_) * dim_);
+  const int st = femgpu_rhs(h_, f.data());
+  if (st != FEMGPU_OK)
+    throw std::runtime_error(std::string("initFlocals: ") + femgpu_last_error(h_));
+}
+
+namespace {
+
 void load_problem(const Ctx &c, const dataKeeper &dk, const SolverOptions &opt) {
   const int dim = dk.get_dim();
   c.check(femgpu_set_mesh(c.h, dim, dk.get_nodesCount(), dk.nodes().data(), dk.get_elementsCount(),
@@ -59,23 +95,35 @@ void load_problem(const Ctx &c, const dataKeeper &dk, const SolverOptions &opt) 
   c.check(femgpu_set_load_convention(c.h, opt.load_convention), "set_load_convention");
 }
 
-void solve_static(const Ctx &c, dataKeeper &dk, double eps, int tol_kind, const SolverOptions &opt,
-                  bool verbose) {
+// The PCG of femgpu_solve_static on h's problem into out (n_nodes x dim).
+void pcg_solve(femgpu_ctx *h, Matrix &out, int64_t nn, int dim, double eps, int tol_kind,
+               int precision, bool verbose) {
   femgpu_solve_opts so;
   femgpu_default_solve_opts(&so);
-  so.precision = opt.precision;
-  so.tol_kind = tol_kind;
+  so.precision = precision;
+  so.tol_kind = precision == FEMGPU_MIXED ? FEMGPU_TOL_REL_RES : tol_kind;
   so.tol = eps;
-  if (so.precision == FEMGPU_MIXED) so.tol_kind = FEMGPU_TOL_REL_RES;
   femgpu_stats st{};
-  c.check(femgpu_solve_static(c.h, &so, dk.displacements().data(), &st), "solve_static");
+  if (out.get_numRows() != size_t(nn) || out.get_numCols() != size_t(dim))
+    out.resize(size_t(nn), size_t(dim));
+  const int rc = femgpu_solve_static(h, &so, out.get_data(), &st);
+  if (rc != FEMGPU_OK)
+    throw std::runtime_error(std::string("solve_static: ") + femgpu_last_error(h) + " (" +
+                             std::to_string(rc) + ")");
   if (verbose)
     std::cout << "PCG: " << st.iterations << " iterations, rel. residual " << st.rel_res
               << ", gamma/gamma0 " << st.gamma_ratio << ", " << st.solve_ms << " ms\n";
-  if (opt.compute_stress) {
-    dk.stresses().assign(size_t(dk.get_elementsCount()) * 3 * (dk.get_dim() - 1), 0.0);
-    c.check(femgpu_stress(c.h, dk.displacements().data(), dk.stresses().data()), "stress");
-  }
+}
+
+// Per-element stress and strain of the solution (the fields of results.vtk).
+void element_results(femgpu_ctx *h, dataKeeper &dk) {
+  const size_t n = size_t(dk.get_elementsCount()) * 3 * (dk.get_dim() - 1);
+  dk.stresses().assign(n, 0.0);
+  dk.strains().assign(n, 0.0);
+  int rc = femgpu_stress(h, dk.displacements().data(), dk.stresses().data());
+  if (rc == FEMGPU_OK) rc = femgpu_strain(h, dk.displacements().data(), dk.strains().data());
+  if (rc != FEMGPU_OK)
+    throw std::runtime_error(std::string("element results: ") + femgpu_last_error(h));
 }
 
 int wavelet_code(const std::string &t) {
@@ -88,22 +136,24 @@ int wavelet_code(const std::string &t) {
 
 void gpuCalculateFEM_EbE_vec(DEVICE_NAME device, dataKeeper &dk, bool PRINT_DEBUG_INFO,
                              const SolverOptions &opt) {
-  require_hip(device);
-  Ctx c(opt.device);
-  load_problem(c, dk, opt);
-  solve_static(c, dk, opt.tol, opt.tol_kind, opt, PRINT_DEBUG_INFO);
+  std::unique_ptr<ElementsData> ed(ElementsData::setElementsData(device, dk, opt));
+  ed->genMask();
+  ed->calculateKlocals();
+  ed->initFlocals(0.f, dk.getWaveletParams());
+  pcg_solve(ed->handle(), *dk.get_displacements(), dk.get_nodesCount(), dk.get_dim(), opt.tol,
+            opt.tol_kind, opt.precision, PRINT_DEBUG_INFO);
+  dk.stresses().clear();
+  dk.strains().clear();
+  if (opt.compute_stress) element_results(ed->handle(), dk);
 }
 
-void gpuPCG_EbE_vec(std::vector<double> &out, DEVICE_NAME device, dataKeeper &dk,
-                    bool /*doAssemblyRes*/, float eps, bool PRINT_DEBUG_INFO,
-                    const SolverOptions &opt) {
+void gpuPCG_EbE_vec(Matrix &displacements, DEVICE_NAME device, dataKeeper &dk,
+                    ElementsData &elemsData, bool /*doAssemblyRes*/, float eps,
+                    bool PRINT_DEBUG_INFO) {
   require_hip(device);
-  Ctx c(opt.device);
-  load_problem(c, dk, opt);
-  SolverOptions o = opt;
-  o.compute_stress = false;
-  solve_static(c, dk, double(eps), FEMGPU_TOL_REF_GAMMA, o, PRINT_DEBUG_INFO);
-  out = dk.displacements();
+  const SolverOptions &o = elemsData.options();
+  pcg_solve(elemsData.handle(), displacements, dk.get_nodesCount(), dk.get_dim(), double(eps),
+            o.tol_kind, o.precision, PRINT_DEBUG_INFO);
 }
 
 void gpuCalculateFEM_DYN(DEVICE_NAME device, dataKeeper &dk, bool PRINT_DEBUG_INFO,
@@ -138,6 +188,8 @@ void gpuCalculateFEM_DYN(DEVICE_NAME device, dataKeeper &dk, bool PRINT_DEBUG_IN
             "set_receivers");
   femgpu_dyn_stats st{};
   c.check(femgpu_dyn_run(c.h, &o, dk.displacements().data(), &st), "dyn_run");
+  dk.stresses().clear();
+  dk.strains().clear();
   if (PRINT_DEBUG_INFO)
     std::cout << "GN22: " << st.steps << " steps" << (m.beta2 == 0 ? " (explicit)" : " (implicit)")
               << ", " << st.pcg_iterations << " PCG iterations, " << st.run_ms << " ms\n";
@@ -174,19 +226,135 @@ void gpuCalculateFEM_DYN(DEVICE_NAME device, dataKeeper &dk, bool PRINT_DEBUG_IN
   }
 }
 
-void WriteResults(dataKeeper &dk) {
+namespace {
+
+// von Mises value of a Voigt vector (xx,yy,xy | xx,yy,zz,xy,yz,xz) with
+// engineering shears for strains as for stresses: the functional
+// sqrt(((a-b)^2 + (b-c)^2 + (c-a)^2) / 2 + 3 (s_xy^2 + s_yz^2 + s_xz^2)),
+// in 2D with the out-of-plane component zero: sqrt(a^2 - a b + b^2 + 3 s^2)
+// (the fixtures' mises_stress and mises_deformation).
+double mises(const double *v, int dim) {
+  if (dim == 2) return std::sqrt(v[0] * v[0] - v[0] * v[1] + v[1] * v[1] + 3.0 * v[2] * v[2]);
+  const double a = v[0] - v[1], b = v[1] - v[2], c = v[2] - v[0];
+  return std::sqrt(0.5 * (a * a + b * b + c * c) + 3.0 * (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
+}
+
+// L2 projection of the element-constant field s_e onto the linear nodal
+// basis: M S = b with the consistent mass matrix M = sum_e |e| (1 + d_ij) /
+// ((d+1)(d+2)) over the element's nodes and b_i = sum_e s_e |e| / (d+1),
+// solved by Jacobi-preconditioned CG (element by element, M is well
+// conditioned) to a relative residual of 1e-14.  Nodes no element touches
+// get 0.
+std::vector<double> project_to_nodes(const dataKeeper &dk, const std::vector<double> &s, int nv,
+                                     int comp) {
+  const int d = dk.get_dim(), npe = d + 1;
+  const int64_t nn = dk.get_nodesCount(), ne = dk.get_elementsCount();
+  const auto &X = dk.nodes();
+  const auto &E = dk.elements();
+  std::vector<double> vol(ne), b(nn, 0.0), diag(nn, 0.0);
+  const double den = double(npe) * double(npe + 1);
+  for (int64_t e = 0; e < ne; ++e) {
+    const int32_t *n = &E[size_t(e) * npe];
+    double v;
+    if (d == 2) {
+      const double *a = &X[size_t(n[0]) * 2], *p = &X[size_t(n[1]) * 2], *q = &X[size_t(n[2]) * 2];
+      v = 0.5 * std::fabs((p[0] - a[0]) * (q[1] - a[1]) - (p[1] - a[1]) * (q[0] - a[0]));
+    } else {
+      const double *a = &X[size_t(n[0]) * 3], *p = &X[size_t(n[1]) * 3], *q = &X[size_t(n[2]) * 3],
+                   *r = &X[size_t(n[3]) * 3];
+      const double e1[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+      const double e2[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+      const double e3[3] = {r[0] - a[0], r[1] - a[1], r[2] - a[2]};
+      v = std::fabs(e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                    e1[2] * (e2[0] * e3[1] - e2[1] * e3[0])) / 6.0;
+    }
+    vol[e] = v;
+    for (int a = 0; a < npe; ++a) {
+      b[n[a]] += s[size_t(e) * nv + comp] * v / npe;
+      diag[n[a]] += 2.0 * v / den;
+    }
+  }
+  auto apply = [&](const std::vector<double> &x, std::vector<double> &y) {
+    std::fill(y.begin(), y.end(), 0.0);
+    for (int64_t e = 0; e < ne; ++e) {
+      const int32_t *n = &E[size_t(e) * npe];
+      double sum = 0.0;
+      for (int a = 0; a < npe; ++a) sum += x[n[a]];
+      const double w = vol[e] / den;
+      for (int a = 0; a < npe; ++a) y[n[a]] += w * (sum + x[n[a]]);
+    }
+  };
+  std::vector<double> x(nn, 0.0), r = b, z(nn), p(nn), q(nn);
+  for (int64_t i = 0; i < nn; ++i) z[i] = diag[i] > 0 ? r[i] / diag[i] : 0.0;
+  p = z;
+  double rz = 0.0, bb = 0.0;
+  for (int64_t i = 0; i < nn; ++i) {
+    rz += r[i] * z[i];
+    bb += b[i] * b[i];
+  }
+  for (int it = 0; it < 1000 && bb > 0; ++it) {
+    apply(p, q);
+    double pq = 0.0;
+    for (int64_t i = 0; i < nn; ++i) pq += p[i] * q[i];
+    if (!(pq > 0)) break;
+    const double al = rz / pq;
+    double rr = 0.0;
+    for (int64_t i = 0; i < nn; ++i) {
+      x[i] += al * p[i];
+      r[i] -= al * q[i];
+      rr += r[i] * r[i];
+    }
+    if (rr <= 1e-28 * bb) break;
+    double rz2 = 0.0;
+    for (int64_t i = 0; i < nn; ++i) {
+      z[i] = diag[i] > 0 ? r[i] / diag[i] : 0.0;
+      rz2 += r[i] * z[i];
+    }
+    const double be = rz2 / rz;
+    rz = rz2;
+    for (int64_t i = 0; i < nn; ++i) p[i] = z[i] + be * p[i];
+  }
+  return x;
+}
+
+void make_results(const dataKeeper &dk, ResultsDataKeeper &R) {
+  const int d = dk.get_dim(), nv = 3 * (d - 1);
+  const int64_t ne = dk.get_elementsCount();
+  R.Stress = dk.stresses();
+  R.Deformation = dk.strains();
+  R.sigma_mises.assign(ne, 0.0);
+  R.epsilon_mises.assign(ne, 0.0);
+  for (int64_t e = 0; e < ne; ++e) {
+    R.sigma_mises[e] = mises(&R.Stress[size_t(e) * nv], d);
+    R.epsilon_mises[e] = mises(&R.Deformation[size_t(e) * nv], d);
+  }
+  if (R.withSmooth)
+    R.SmoothStress = project_to_nodes(dk, R.Stress, nv, 0);  // "xx" (femfunc.cpp:1248-1255)
+  else
+    R.SmoothStress.assign(size_t(dk.get_nodesCount()), 0.0);
+}
+
+}  // namespace
+
+void WriteResults(dataKeeper &dk, ResultsDataKeeper &RESdata) {
   std::error_code ec;
   std::filesystem::create_directories(
       std::filesystem::path(dk.getDataPaths().output_vtk).parent_path(), ec);
+  const size_t nv = size_t(3 * (dk.get_dim() - 1));
+  const bool have = !dk.stresses().empty() &&
+                    dk.stresses().size() == size_t(dk.get_elementsCount()) * nv &&
+                    dk.strains().size() == dk.stresses().size();
+  if (have) make_results(dk, RESdata);
   if (dk.get_dim() == 2)
-    MakeVTKfile2D(dk.getDataPaths().output_vtk, dk);
+    MakeVTKfile2D(dk.getDataPaths().output_vtk, dk, have ? &RESdata : nullptr);
   else
-    MakeVTKfile3D(dk.getDataPaths().output_vtk, dk);
+    MakeVTKfile3D(dk.getDataPaths().output_vtk, dk, have ? &RESdata : nullptr);
 }
 
 namespace {
 
-void write_vtk(const std::string &path, const dataKeeper &dk, bool is2d) {
+void write_vtk(const std::string &path, const dataKeeper &dk, const ResultsDataKeeper *R,
+               bool is2d) {
   std::ofstream out(path);
   if (!out) throw std::runtime_error("cannot write " + path);
   out << std::setprecision(9);
@@ -221,39 +389,52 @@ void write_vtk(const std::string &path, const dataKeeper &dk, bool is2d) {
     for (int k = 0; k < d; ++k) s += U[i * d + k] * U[i * d + k];
     out << std::sqrt(s) << "\n";
   }
-  const std::vector<double> &S = const_cast<dataKeeper &>(dk).stresses();
+  if (!R) return;
+  // the field layout of the reference's fixtures (testing/*_test/results.vtk)
   const int nv = 3 * (d - 1);
-  if (S.size() == size_t(ne) * nv) {
-    // the per-element tensor block of the legacy fixtures (Voigt -> 3x3)
-    out << "\nCELL_DATA " << ne << "\nTENSORS stress float\n";
-    for (int64_t e = 0; e < ne; ++e) {
-      const double *s = &S[e * nv];
-      if (is2d) {
-        out << s[0] << " " << s[2] << " 0\n" << s[2] << " " << s[1] << " 0\n0 0 0\n";
-      } else {
-        out << s[0] << " " << s[3] << " " << s[5] << "\n"
-            << s[3] << " " << s[1] << " " << s[4] << "\n"
-            << s[5] << " " << s[4] << " " << s[2] << "\n";
-      }
+  out << "\nSCALARS SmoothStress float\nLOOKUP_TABLE default\n";
+  for (int64_t i = 0; i < nn; ++i) out << R->SmoothStress[i] << "\n";
+  // symmetric 3x3 tensor of a Voigt vector; shear factor 1 (stress) or 1/2
+  // (engineering strain -> tensor strain)
+  auto tensor = [&](const double *s, double h) {
+    if (is2d) {
+      out << s[0] << " " << h * s[2] << " 0\n" << h * s[2] << " " << s[1] << " 0\n0 0 0\n";
+    } else {
+      out << s[0] << " " << h * s[3] << " " << h * s[5] << "\n"
+          << h * s[3] << " " << s[1] << " " << h * s[4] << "\n"
+          << h * s[5] << " " << h * s[4] << " " << s[2] << "\n";
     }
-    out << "\nSCALARS mises_stress float\nLOOKUP_TABLE default\n";
+  };
+  const int ixy = is2d ? 2 : 3;
+  out << "\nCELL_DATA " << ne << "\nTENSORS stress float\n";
+  for (int64_t e = 0; e < ne; ++e) tensor(&R->Stress[size_t(e) * nv], 1.0);
+  out << "\nSCALARS XY_STRESS float\nLOOKUP_TABLE default\n";
+  for (int64_t e = 0; e < ne; ++e) out << R->Stress[size_t(e) * nv + ixy] << "\n";
+  out << "\nSCALARS XY_DEFORM float\nLOOKUP_TABLE default\n";
+  for (int64_t e = 0; e < ne; ++e) out << R->Deformation[size_t(e) * nv + ixy] << "\n";
+  out << "\nSCALARS mises_stress float\nLOOKUP_TABLE default\n";
+  for (int64_t e = 0; e < ne; ++e) out << R->sigma_mises[e] << "\n";
+  if (is2d) {
+    out << "\nVECTORS deformation float\n";
     for (int64_t e = 0; e < ne; ++e) {
-      const double *s = &S[e * nv];
-      double vm;
-      if (is2d)
-        vm = std::sqrt(s[0] * s[0] - s[0] * s[1] + s[1] * s[1] + 3 * s[2] * s[2]);
-      else
-        vm = std::sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) +
-                              (s[2] - s[0]) * (s[2] - s[0])) +
-                       3 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
-      out << vm << "\n";
+      const double *s = &R->Deformation[size_t(e) * nv];
+      out << s[0] << " " << s[1] << " " << s[2] << "\n";
     }
+  } else {
+    out << "\nTENSORS deformation float\n";
+    for (int64_t e = 0; e < ne; ++e) tensor(&R->Deformation[size_t(e) * nv], 0.5);
   }
+  out << "\nSCALARS mises_deformation float\nLOOKUP_TABLE default\n";
+  for (int64_t e = 0; e < ne; ++e) out << R->epsilon_mises[e] << "\n";
 }
 
 }  // namespace
 
-void MakeVTKfile2D(const std::string &path, const dataKeeper &dk) { write_vtk(path, dk, true); }
-void MakeVTKfile3D(const std::string &path, const dataKeeper &dk) { write_vtk(path, dk, false); }
+void MakeVTKfile2D(const std::string &path, const dataKeeper &dk, const ResultsDataKeeper *res) {
+  write_vtk(path, dk, res, true);
+}
+void MakeVTKfile3D(const std::string &path, const dataKeeper &dk, const ResultsDataKeeper *res) {
+  write_vtk(path, dk, res, false);
+}
 
 }  // namespace femproject

@@ -10,6 +10,7 @@
 //          --loads reference|physical  --mass reference|physical
 //          --implicit reference|physical  --no-stress  --device N
 //          --dry-run (read the problem, print its summary, no GPU)
+//          --smooth (results.vtk SmoothStress: nodal L2 projection of sigma_xx)
 //          --relax (endtime < 0: step until max |v| < |endtime|)
 //          --damping (apply the config's damping_alpha / damping_beta)
 //          --receivers n1,n2,... --receiver-axis 0|1|2
@@ -37,6 +38,7 @@ bool ends_with(const std::string &s, const std::string &t) {
 }
 
 bool g_dry_run = false;
+bool g_smooth = false;
 
 int parse_opts(int argc, char **argv, int first, SolverOptions &o, bool &tol_set) {
   for (int i = first; i < argc; ++i) {
@@ -62,6 +64,8 @@ int parse_opts(int argc, char **argv, int first, SolverOptions &o, bool &tol_set
       o.implicit_mode = val() == "physical" ? FEMGPU_IMPLICIT_PHYSICAL : FEMGPU_IMPLICIT_REFERENCE;
     } else if (a == "--no-stress") {
       o.compute_stress = false;
+    } else if (a == "--smooth") {
+      g_smooth = true;
     } else if (a == "--dry-run") {
       g_dry_run = true;
     } else if (a == "--device") {
@@ -113,7 +117,9 @@ int run(dataKeeper &dk, const SolverOptions &opt, bool verbose) {
     gpuCalculateFEM_DYN(HIP, dk, verbose, opt);
   else
     gpuCalculateFEM_EbE_vec(HIP, dk, verbose, opt);
-  WriteResults(dk);
+  ResultsDataKeeper RESdata(false, false, false, dk.get_nodesCount());  // main.cpp:30-33
+  RESdata.withSmooth = g_smooth;
+  WriteResults(dk, RESdata);
   if (verbose) std::cout << "wrote " << dk.getDataPaths().output_vtk << "\n";
   return 0;
 }

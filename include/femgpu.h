@@ -131,6 +131,19 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
  * No counterpart in the reference (its kernels use 256 threads and one
  * thread per element, fem_utils_kernels.cu:3). */
 int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads);
+/* Autotuner (no counterpart in the reference, whose kernels use fixed
+ * 256-thread blocks and one element per thread): for the current problem
+ * (mesh, material, constraints set), times a short PCG run (64 iterations on
+ * a unit right-hand side; FP64, or FP32 for FEMGPU_FP32 / FEMGPU_MIXED whose
+ * inner iterations are FP32) under each candidate patch tuning (256, 512 or
+ * 1024 threads per patch with 2 to 3 elements per thread; the candidates
+ * are planned on host threads in parallel), re-plans the mesh with the
+ * fastest and keeps it as the context's tuning (as femgpu_set_tuning).  The
+ * choice is cached for the process per mesh (hash of coordinates and
+ * connectivity) and arithmetic, so a later call on the same mesh only
+ * re-plans.  Outputs may be NULL. */
+int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *patch_threads,
+                    double *ms_per_iteration);
 /* The patch plan femgpu_set_mesh would build for this mesh and tuning,
  * computed on the host (needs no device): info[0] patches, [1] nodes owned
  * by one patch, [2] interface nodes, [3] interface partial-sum slots,
@@ -274,6 +287,12 @@ int femgpu_batch_solve(femgpu_ctx *ctx, int n_rhs, const double *f, const femgpu
  * u: n_nodes x dim; sigma_out: n_elems x 3(dim-1) Voigt (xx,yy,xy | xx,yy,zz,
  * xy,yz,xz), user element order. */
 int femgpu_stress(femgpu_ctx *ctx, const double *u, double *sigma_out);
+/* Per-element engineering strain eps = B u_e (xx,yy,2xy | xx,yy,zz,2xy,2yz,
+ * 2xz), n_elems x 3(dim-1), user element order.  The reference's live code
+ * has none; its earlier post-processing computed this `Deformation = B
+ * delta` per element (deprecated/femfunc/femfunc.cpp:42-99) for the legacy
+ * results.vtk fields (deformation, XY_DEFORM, mises_deformation). */
+int femgpu_strain(femgpu_ctx *ctx, const double *u, double *eps_out);
 /* Element strain energies e_e = u_e' K_e u_e (n_elems). */
 int femgpu_element_energy(femgpu_ctx *ctx, const double *u, double *e_out);
 

@@ -100,3 +100,47 @@ def test_run_fem_parses_raw_mesh(tmp_path):
     prep = tmp_path / "work" / "prepared_meshes" / "2D" / "9task_3"
     for f in ("nodes.txt", "elements.txt", "constraints.txt", "loads.txt", "stress.txt"):
         assert (prep / f).exists()
+
+
+REFERENCE_SHAPED_MAIN = r'''
+// The call sequence of the reference's fem/src/main.cpp:10-36 with the
+// device set to HIP, against this build's headers (names at global scope).
+#include <femfunc.h>
+#include <datakeeper.h>
+#include <memory>
+int main(int argc, char *argv[]) {
+  DEVICE_NAME deviceType = HIP;
+  dataKeeper dk(argv[1], argv[2]);
+  if (dk.getTaskType() == "static")
+    gpuCalculateFEM_EbE_vec(deviceType, dk, true);
+  else
+    gpuCalculateFEM_DYN(deviceType, dk, true);
+  ResultsDataKeeper RESdata2(false, false, false, dk.get_nodesCount());
+  WriteResults(dk, RESdata2);
+  // the PCG entry point on its own (femfunc.cpp:5-20 call sequence)
+  std::unique_ptr<ElementsData> ed(ElementsData::setElementsData(deviceType, dk));
+  ed->genMask();
+  ed->calculateKlocals();
+  ed->initFlocals(0.f, dk.getWaveletParams());
+  Matrix *u = dk.get_displacements();
+  gpuPCG_EbE_vec(*u, deviceType, dk, *ed, true, 1e-8f, true);
+  return u->get_numRows() == size_t(dk.get_nodesCount()) ? 0 : 1;
+}
+'''
+
+
+def test_reference_shaped_main_compiles_and_links(tmp_path, fem_main):
+    """A caller written against the reference's femfunc.h / datakeeper.h
+    (global names, reference signatures of gpuPCG_EbE_vec and WriteResults)
+    compiles and links against this build with only the device changed."""
+    src = tmp_path / "ref_main.cpp"
+    src.write_text(REFERENCE_SHAPED_MAIN)
+    pkg = os.path.join(ROOT, "cudafemproject_amd")
+    fp = os.path.join(pkg, "femproject")
+    srcs = [os.path.join(fp, f) for f in sorted(os.listdir(fp))
+            if f.endswith(".cpp") and f != "main.cpp"]
+    exe = tmp_path / "ref_main"
+    r = subprocess.run(["g++", "-std=c++17", "-O0", f"-I{fp}", f"-I{os.path.join(ROOT, 'include')}",
+                        str(src), *srcs, "-o", str(exe), f"-L{pkg}", "-lfemgpu",
+                        f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

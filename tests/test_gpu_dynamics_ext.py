@@ -250,3 +250,81 @@ def test_fem_main_writes_receivers_and_segy(tmp_path, fem_main):
         o = 3600 + t * (240 + 4 * ns) + 240
         tr = np.frombuffer(b[o:o + 4 * ns], ">f4")
         np.testing.assert_allclose(tr, vals[:, 1 + t], rtol=1e-6, atol=1e-30)
+
+
+# ---------------------------------------------------------------------------
+# Restatements: receiver traces against the oracle's GN22 stepper, damped
+# stepping against a dense GN22 with Rayleigh damping.
+
+@pytest.mark.parametrize("tuning", [(0, 0), (48, 256)])
+def test_traces_vs_oracle(ctx, tuning):
+    """Displacement traces (every 4th step) equal the oracle's restatement of
+    gpuCalculateFEM_DYN (femfunc.cpp:130-381) stopped at each sampled step
+    (reference conventions, Ricker wavelet): <= 1e-11 relative per sample,
+    on one patch and on 48-element patches (receivers on interface nodes)."""
+    import oracle as O
+    pb = P.column_problem(2, 2, 8, 0.1, 0.1, 0.8, pressure=1.0, convention=F.LOADS_REFERENCE)
+    rho, dt, every, n = 1000.0, 1e-4, 4, 24
+    wav = {"type": "ricker", "freq": 50.0, "timeshift": 0.0, "ampl": 1000.0}
+    recv = np.array([3, 20, pb.n_nodes // 2, pb.n_nodes - 1], np.int32)
+    ctx.set_tuning(*tuning)
+    try:
+        ctx.set_problem(pb)
+        ctx.set_receivers(recv, F.RECV_DISPLACEMENT, every)
+        ctx.dyn_run(rho=rho, beta1=0.5, beta2=0.0, dt=dt, endtime=(n + 0.5) * dt,
+                    wavelet_type=F.WAVELET_RICKER, freq=50.0, ampl=1000.0)
+        tr = ctx.traces()
+        ctx.set_receivers([], F.RECV_DISPLACEMENT, 1)
+    finally:
+        ctx.set_tuning(0, 0)
+    assert tr.shape == (n // every, len(recv), 3)
+    for k in range(1, n // every + 1):
+        s, uo, steps = O.dyn_solve(pb, rho, 0.5, 0.0, dt, (k * every + 0.5) * dt, wav,
+                                   np.float64, compat_scramble=False)
+        assert s == 0 and steps == k * every
+        assert rel(tr[k - 1], uo[recv]) < 1e-11, k
+
+
+def _dense_damped(pb, rho, beta1, beta2, dt, nsteps, f, alpha, beta):
+    """Dense GN22 with Rayleigh damping C = alpha M + beta K, lumped mass
+    rho V/4 (3D): M a' + C v' + K d' = f with d~ = d + dt v + (1-beta2) dt^2/2 a,
+    v~ = v + (1-beta1) dt a, v' = v~ + beta1 dt a', d' = d~ + beta2 dt^2/2 a'."""
+    from helpers import dense_K
+    K = dense_K(pb)
+    m = np.zeros(pb.n_dof)
+    for e in pb.elems:
+        v = abs(np.linalg.det(np.c_[np.ones(4), pb.nodes[e]])) / 6 / 4
+        for a in e:
+            m[3 * a:3 * a + 3] += rho * v
+    M = np.diag(m)
+    C = alpha * M + beta * K
+    A = M + beta1 * dt * C + 0.5 * beta2 * dt * dt * K
+    x, v, a = np.zeros(pb.n_dof), np.zeros(pb.n_dof), np.zeros(pb.n_dof)
+    for _ in range(nsteps):
+        xt = x + dt * v + 0.5 * (1 - beta2) * dt * dt * a
+        vt = v + (1 - beta1) * dt * a
+        a = np.linalg.solve(A, f - C @ vt - K @ xt)
+        v = vt + beta1 * dt * a
+        x = xt + 0.5 * beta2 * dt * dt * a
+    return x.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("beta2,alpha,beta", [(0.0, 300.0, 0.0), (0.6, 300.0, 0.0),
+                                              (0.6, 0.0, 2e-5), (0.6, 300.0, 2e-5)])
+def test_rayleigh_damping_vs_dense(ctx, beta2, alpha, beta):
+    """Mass- and stiffness-proportional damping (damping_alpha, damping_beta,
+    femfunc.cpp:137-150,190-200,266-274), explicit (beta = 0) and implicit
+    physical mode, FP64 state: <= 1e-8 relative to the dense restatement."""
+    pb = P.column_problem(1, 1, 4, 0.1, 0.1, 0.4, convention=F.LOADS_PHYSICAL, pressure=1e4)
+    ctx.set_problem(pb)
+    f = ctx.rhs().ravel()
+    rho, dt, T = 1000.0, 2e-6, 2e-4
+    u, st = ctx.dyn_run(rho=rho, beta1=0.5, beta2=beta2, dt=dt, endtime=T,
+                        mass_convention=F.MASS_PHYSICAL, implicit_mode=F.IMPLICIT_PHYSICAL,
+                        pcg_eps=1e-26, damping_alpha=alpha, damping_beta=beta)
+    nsteps = int(T / dt)
+    assert st["steps"] == nsteps
+    ud = _dense_damped(pb, rho, 0.5, beta2, dt, nsteps, f, alpha, beta)
+    und = _dense_damped(pb, rho, 0.5, beta2, dt, nsteps, f, 0.0, 0.0)
+    assert rel(ud, und) > 1e-4          # the damping changes the trajectory
+    assert rel(u, ud) < 1e-8
