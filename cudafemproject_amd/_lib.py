@@ -89,6 +89,7 @@ SIGNATURES = {
     "femgpu_set_load_convention": (C.c_int, [_P, C.c_int]),
     "femgpu_set_extra_rhs": (C.c_int, [_P, _D]),
     "femgpu_matvec": (C.c_int, [_P, C.c_int, _D, _D]),
+    "femgpu_matvec_noalloc": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p]),
     "femgpu_diag": (C.c_int, [_P, _D]),
     "femgpu_rhs": (C.c_int, [_P, _D]),
     "femgpu_matvec_repeat": (C.c_int, [_P, C.c_int, C.c_int]),

@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function"]
 
-LIB_SOURCES = ["kernels.hip", "cg1.hip", "batch.hip", "batch_ls.hip", "femgpu_api.cpp",
+LIB_SOURCES = ["kernels.hip", "cg1.hip", "batch.hip", "batch_ls.hip", "noalloc.hip", "femgpu_api.cpp",
                "batch.cpp", "mesh_plan.cpp", "segy.cpp"]
 
 

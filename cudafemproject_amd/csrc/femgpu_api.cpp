@@ -24,6 +24,7 @@
 #include "cg1.h"
 #include "femgpu_internal.h"
 #include "kernels.h"
+#include "noalloc.h"
 
 using femgpu::CgScalars;
 using femgpu::DevMesh;
@@ -126,6 +127,11 @@ struct femgpu_ctx {
   std::map<std::string, hipGraphExec_t> graphs;
   uint64_t buf_gen = 0;
   uint64_t graph_gen = 0;
+
+  // femgpu_matvec_noalloc: this device's fixed scratch array (device
+  // address from hipGetSymbolAddress) and the internal -> user node map
+  void *scratch = nullptr;
+  const int32_t *d_node_perm = nullptr;
 };
 
 namespace femgpu {
@@ -741,6 +747,31 @@ int true_rel_res(femgpu_ctx *ctx, const double *f, const double *x, double *out)
 
 }  // namespace
 
+namespace {
+
+// femgpu_matvec_noalloc in arithmetic T on the scratch array.
+template <typename T>
+int matvec_noalloc_t(femgpu_ctx *ctx, const double *x, double *y) {
+  const int64_t nd = ctx->nn * ctx->dim;
+  const size_t vec = (size_t(nd) * sizeof(T) + 255) & ~size_t(255);
+  const size_t need = 2 * vec + (size_t(ctx->plan.n_slots) * ctx->dim + 1) * sizeof(T);
+  if (need > femgpu::NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, FEMGPU_E_ARG,
+                "problem too large for femgpu_matvec_noalloc: needs " + std::to_string(need) +
+                    " bytes of scratch, the limit is " +
+                    std::to_string(femgpu::NOALLOC_SCRATCH_BYTES));
+  unsigned char *base = static_cast<unsigned char *>(ctx->scratch);
+  T *xi = reinterpret_cast<T *>(base);
+  T *yi = reinterpret_cast<T *>(base + vec);
+  T *part = reinterpret_cast<T *>(base + 2 * vec);
+  femgpu::launch_gather_user<T>(ctx->nn, ctx->dim, ctx->d_node_perm, x, xi, ctx->stream);
+  femgpu::launch_matvec<T>(ctx->dm, xi, yi, part, ctx->stream);
+  femgpu::launch_scatter_user<T>(ctx->nn, ctx->dm.n_active, ctx->dim, ctx->d_node_perm, yi, y,
+                                 ctx->stream);
+  HIPCHK(hipGetLastError());
+  return FEMGPU_OK;
+}
+}  // namespace
 
 static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char *f_name,
                       double *u_out, femgpu_stats *stats);
@@ -786,7 +817,11 @@ int upload_plan(femgpu_ctx *ctx) {
       (st = upload(ctx, "slot_start", P.slot_start, &ss)) ||
       (st = upload(ctx, "elems_int", P.elems_int, &ei)))
     return st;
-
+  {
+    int32_t *np_;
+    if ((st = upload(ctx, "node_perm", P.node_perm, &np_))) return st;
+    ctx->d_node_perm = np_;
+  }
   dm.patches = pp;
   dm.erec = rec;
   dm.sh_gid = shg;
@@ -841,6 +876,7 @@ int femgpu_create(femgpu_ctx **out, int device_ordinal) {
   femgpu_ctx *ctx = c.get();
   ctx->device = device_ordinal;
   HIPCHK(hipSetDevice(device_ordinal));
+  HIPCHK(femgpu::noalloc_scratch_address(device_ordinal, &ctx->scratch));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   ctx->own_stream = true;
   HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_sc), 4 * sizeof(CgScalars), 0));
@@ -1271,6 +1307,23 @@ int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y) {
   return FEMGPU_OK;
 }
 
+
+int femgpu_matvec_noalloc(femgpu_ctx *ctx, int precision, const double *x_dev, double *y_dev) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!x_dev || !y_dev) return fail(ctx, FEMGPU_E_ARG, "null vector");
+  if (!ctx->scratch) return fail(ctx, FEMGPU_E_STATE, "no scratch address for this device");
+  // the operator (material, constraint mask, assembled diagonal of the
+  // constrained rows) is prepared by the allocating entry points
+  if (ctx->dirty_mat || ctx->dirty_fix || ctx->dirty_diag)
+    return fail(ctx, FEMGPU_E_STATE,
+                "operator not prepared since the last change of mesh, material or constraints: "
+                "call femgpu_diag, femgpu_matvec or a solve once first");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (precision == FEMGPU_FP64) return matvec_noalloc_t<double>(ctx, x_dev, y_dev);
+  if (precision == FEMGPU_FP32) return matvec_noalloc_t<float>(ctx, x_dev, y_dev);
+  return fail(ctx, FEMGPU_E_ARG, "precision must be FP64 or FP32");
+}
 
 int femgpu_matvec_repeat(femgpu_ctx *ctx, int precision, int reps) {
   int st = need_mesh(ctx);

@@ -177,6 +177,23 @@ int femgpu_set_extra_rhs(femgpu_ctx *ctx, const double *f);
  * y = K x is one on-the-fly EbE product into assembled global vectors.
  * x, y: n_nodes x dim host arrays in user node order. */
 int femgpu_matvec(femgpu_ctx *ctx, int precision, const double *x, double *y);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): y = K x like femgpu_matvec, on caller-owned DEVICE
+ * arrays x_dev, y_dev (n_nodes x dim FP64, user node order; y is 0 at nodes
+ * no element references), `precision` the arithmetic (FEMGPU_FP64 or
+ * FEMGPU_FP32).  Allocates nothing: the internal-order copies of x and y and
+ * the interface partial sums live in one fixed scratch array of 32 MiB per
+ * GPU, held until the process ends; a problem needing more (2 x n_nodes x
+ * dim values rounded up to 256 bytes each plus dim x interface slots + 1
+ * values; the 64^3-hex tet cube needs 20.8 MB in FP64) is rejected with
+ * FEMGPU_E_ARG.  Every call on one GPU shares that array, so calls on one
+ * GPU that overlap (two contexts, two streams) race; calls on one context
+ * are ordered by its stream.  Enqueued on the context stream, no
+ * synchronisation.  The operator must be prepared by an allocating call
+ * (femgpu_diag, femgpu_matvec or a solve) after the last change of mesh,
+ * material or constraints, else FEMGPU_E_STATE. */
+int femgpu_matvec_noalloc(femgpu_ctx *ctx, int precision, const double *x_dev, double *y_dev);
 /* Assembled diagonal of K (Matrix::getDiagonal + reductionWithMask,
  * cpu_matrix.cpp:296-305; femfunc.cpp:10, :41). */
 int femgpu_diag(femgpu_ctx *ctx, double *d);
