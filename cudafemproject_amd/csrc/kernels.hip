@@ -227,9 +227,7 @@ void k_ebe_patch(PatchArgs<T> a) {
     } else if constexpr (MODE == 1) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) xin[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
-      // FP32: p is stored after the assembly (the phase barriers then do not
-      // wait for the writes); FP64 at once (registers)
-      if (sizeof(T) == 8 && own) {
+      if (own) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
       }
@@ -279,13 +277,6 @@ void k_ebe_patch(PatchArgs<T> a) {
   // node's mass term of p'(cK K + cM M) p.)
   if constexpr (MODE == 1) {
     if (has_node && !own && (a.sh.m || fm)) first = slot == a.slot_start[gid - a.n_owned];
-    if (has_node && !own && first && a.sh.m) {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        const double pv = double(xin[c]);
-        acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * pv * pv;
-      }
-    }
   } else if constexpr (MODE == 0) {
     if (has_node && !own && fm) first = slot == a.slot_start[gid - a.n_owned];
   } else if constexpr (MODE == 3) {
@@ -300,6 +291,34 @@ void k_ebe_patch(PatchArgs<T> a) {
   // element by element: sum_e p_e'K_e p_e regrouped, with fewer operations.
   assemble<T, DIM, PT, MODE == 2, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0,
                                          b1, acc, acc_en, double(cK));
+
+  if constexpr (MODE == 0 || MODE == 1) {
+    if (has_node) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) xin[c] = s_x[k * DIM + c];
+      if (fm) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          if ((fm >> c) & 1u) {
+            if constexpr (MODE == 0)
+              xin[c] = a.x[gid * DIM + c];
+            else if (own)
+              xin[c] = a.p[gid * DIM + c];
+            else
+              xin[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
+          }
+      }
+    }
+    if constexpr (MODE == 1) {
+      if (has_node && !own && first && a.sh.m) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          const double pv = double(xin[c]);
+          acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * pv * pv;
+        }
+      }
+    }
+  }
 
   if (has_node) {
     // rows of constrained components: the assembled diagonal only
@@ -341,10 +360,6 @@ void k_ebe_patch(PatchArgs<T> a) {
           acc_en += double(xin[c]) * double(v);
         }
         a.y[gid * DIM + c] = v;
-      }
-      if constexpr (MODE == 1 && sizeof(T) == 4) {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
       }
     } else {
 #pragma unroll

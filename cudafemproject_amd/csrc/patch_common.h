@@ -95,15 +95,19 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
                                          const T *esc, int n_elem, T lam, T mu, int b0, int b1,
                                          T *acc, double &en_acc, double en_scale) {
   constexpr int NPE = DIM + 1, NDE = DIM * NPE;
-  T F[EPT][NDE];
-  int q[EPT][4];
+  // Forces of corners 1..DIM per element (corner 0 of the force-free forms
+  // is -(sum of the others) and is formed again in phase C; the diagonal
+  // keeps all corners).  Few live registers here keep eight 256-thread
+  // workgroups resident per CU in FP32.
+  constexpr int NK = DIAG ? NDE : NDE - DIM;
+  T F[EPT][NK];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * PT;
-    int l[4];
-    decode_rec(rec[e], l, q[e]);
     if (i >= n_elem) continue;
-    T X[NDE], U[NDE];
+    int l[4], qd[4];
+    decode_rec(rec[e], l, qd);
+    T X[NDE], U[NDE], Fe[NDE];
     // byte offset of the corner's node row, shared by s_c and s_x
 #pragma unroll
     for (int n = 0; n < NPE; ++n) {
@@ -117,19 +121,38 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
       }
     }
     if constexpr (DIAG) {
-      elem::elem_diag<T, DIM>(X, lam, mu, esc[e], F[e]);
+      elem::elem_diag<T, DIM>(X, lam, mu, esc[e], Fe);
     } else {
-      const T en = elem::elem_force_free<T, DIM>(X, U, lam, mu, esc[e], F[e]);
+      const T en = elem::elem_force_free<T, DIM>(X, U, lam, mu, esc[e], Fe);
       if constexpr (EN) en_acc += en_scale * double(en);
     }
+#pragma unroll
+    for (int k = 0; k < NK; ++k) F[e][k] = Fe[NDE - NK + k];
+  }
+  int q[EPT][4];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    int l[4];
+    decode_rec(rec[e], l, q[e]);
   }
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e)
-      if (threadIdx.x + e * PT < n_elem)
+      if (threadIdx.x + e * PT < n_elem) {
+        if constexpr (DIAG) {
 #pragma unroll
-        for (int n = 0; n < NPE; ++n) s_res[q[e][n]] = F[e][n * DIM + c];
+          for (int n = 0; n < NPE; ++n) s_res[q[e][n]] = F[e][n * DIM + c];
+        } else {
+          // f_0 = -(f_1 + f_2 + ...), the order of elem_force_free
+          T f0 = F[e][c];
+#pragma unroll
+          for (int n = 2; n < NPE; ++n) f0 += F[e][(n - 1) * DIM + c];
+          s_res[q[e][0]] = -f0;
+#pragma unroll
+          for (int n = 1; n < NPE; ++n) s_res[q[e][n]] = F[e][(n - 1) * DIM + c];
+        }
+      }
     __syncthreads();
     using V4 = typename Vec4<T>::type;
     const V4 *r4 = reinterpret_cast<const V4 *>(s_res);
@@ -155,24 +178,26 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
   }
 }
 
-// acc += the slots [j0, j1) of an interface node, in slot order, two loads
-// in flight per step.
+// acc += the slots [j0, j1) of an interface node, in slot order.  A node has
+// one slot per patch touching it (two on a face shared by two patches, four
+// on an edge, up to eight at a corner of a structured patch grid): the loads
+// of up to eight slots are issued together, then added one by one in slot
+// order; longer lists continue in steps of eight.
 template <typename T, int DIM>
 __device__ __forceinline__ void add_slots(const T *partial, int j0, int j1, T *acc) {
-  int j = j0;
-  for (; j + 2 <= j1; j += 2) {
-    T u[DIM], w[DIM];
+  for (int j = j0; j < j1; j += 8) {
+    const int n = j1 - j < 8 ? j1 - j : 8;
+    T u[8][DIM];
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      u[c] = partial[int64_t(j) * DIM + c];
-      w[c] = partial[int64_t(j + 1) * DIM + c];
-    }
+    for (int r = 0; r < 8; ++r)
+      if (r < n)
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] = (acc[c] + u[c]) + w[c];
-  }
-  if (j < j1) {
+        for (int c = 0; c < DIM; ++c) u[r][c] = partial[int64_t(j + r) * DIM + c];
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) acc[c] += partial[int64_t(j) * DIM + c];
+    for (int r = 0; r < 8; ++r)
+      if (r < n)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) acc[c] += u[r][c];
   }
 }
 
