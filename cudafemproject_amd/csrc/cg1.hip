@@ -157,13 +157,9 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   int b0 = 0, b1 = 0;
   unsigned fm = 0;
   T u[DIM];
-  // this node's new state (first patch only); FP32: stored after the
-  // assembly so that the phase barriers do not wait for these writes (FP64:
-  // at once, the registers are needed by the element math)
-  constexpr bool LATE = sizeof(T) == 4;
-  T np_[DIM], ns_[DIM], nx_[DIM], nr_[DIM];
+  constexpr bool REREAD = sizeof(T) == 4;
 #pragma unroll
-  for (int c = 0; c < DIM; ++c) u[c] = np_[c] = ns_[c] = nx_[c] = nr_[c] = T(0);
+  for (int c = 0; c < DIM; ++c) u[c] = T(0);
   double acc_g = 0.0, acc_d = 0.0, acc_q = 0.0;
   if (has_node) {
     int s0 = 0, s1 = 0;
@@ -221,19 +217,16 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
         rn[c] = rv[c] - alpha * sn;
         u[c] = dv[c] * rn[c];
         const T pn = dv[c] * rv[c] + beta * pv[c];
-        np_[c] = pn;
-        ns_[c] = sn;
-        nx_[c] = xv[c] + alpha * pn;
-        nr_[c] = rn[c];
-        if (!LATE && first) {
-          S.p[out][o + c] = np_[c];
-          S.s[out][o + c] = ns_[c];
-          S.x[out][o + c] = nx_[c];
-          S.r[out][o + c] = nr_[c];
+        // this node's new state, stored by its first patch
+        if (first) {
+          S.p[out][o + c] = pn;
+          S.s[out][o + c] = sn;
+          S.x[out][o + c] = xv[c] + alpha * pn;
+          S.r[out][o + c] = rn[c];
         }
       }
     }
-    if (first) {
+    if (!REREAD && first) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         acc_g += double(rn[c]) * double(u[c]);
@@ -251,6 +244,25 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   assemble<T, DIM, PT, false, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0, b1, acc,
                                      acc_d, 1.0);
   if (has_node) {
+    // FP32: u and the new residual again (not held in registers through
+    // the assembly, which keeps eight workgroups per CU): u from the LDS
+    // copy; the residual (needed by the node's first patch, for r'u and r'r,
+    // and for u = M^-1 r in constrained components) from where phase A
+    // stored it.  FP64 keeps them in registers.
+    const T *rnew = INIT ? S.r[0] : S.r[out];
+    if (REREAD) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) u[c] = s_x[k * DIM + c];
+    }
+    if (REREAD && first) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const T rn = rnew[gid * DIM + c];
+        if ((fm >> c) & 1u) u[c] = a.invD[gid * DIM + c] * rn;
+        acc_g += double(rn) * double(u[c]);
+        acc_q += double(rn) * double(rn);
+      }
+    }
     // constrained rows: the assembled diagonal only (applyConstraints,
     // cpu_fem_utils.cpp:461-476), written by the node's first patch
     if (fm) {
@@ -266,15 +278,6 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
     } else {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) S.slot[out][slot * DIM + c] = acc[c];
-    }
-    if (!INIT && LATE && first) {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        S.p[out][gid * DIM + c] = np_[c];
-        S.s[out][gid * DIM + c] = ns_[c];
-        S.x[out][gid * DIM + c] = nx_[c];
-        S.r[out][gid * DIM + c] = nr_[c];
-      }
     }
   }
   block_sum3<PT>(acc_g, acc_d, acc_q, s_red);

@@ -486,10 +486,22 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   CgScalars *sc = a.sc;
   const int64_t stride = int64_t(gridDim.x) * UPT;
   const int64_t i0 = int64_t(blockIdx.x) * UPT + threadIdx.x;
+  // The per-patch p'q partials are loaded first (up to PQ_REG per thread
+  // into registers), then this thread's first node: the partials return
+  // first and their reduction does not wait for the node loads.
+  constexpr int PQ_REG = 8;
+  double pqv[PQ_REG];
+#pragma unroll
+  for (int j = 0; j < PQ_REG; ++j) {
+    const int i = threadIdx.x + j * UPT;
+    pqv[j] = i < a.np ? a.ppq[i] : 0.0;
+  }
   UpdNode<T, DIM> n0;
   if (i0 < a.n_active) upd_load<T, DIM>(a, i0, n0);
   double pq = 0.0;
-  for (int i = threadIdx.x; i < a.np; i += UPT) pq += a.ppq[i];
+#pragma unroll
+  for (int j = 0; j < PQ_REG; ++j) pq += pqv[j];
+  for (int i = threadIdx.x + PQ_REG * UPT; i < a.np; i += UPT) pq += a.ppq[i];
   if (sc->done) return;
   pq = block_sum<UPT>(pq, s_red);
   const double gam = sc->g[a.parity];

@@ -178,26 +178,24 @@ __device__ __forceinline__ void assemble(const T *s_c, const T *s_x, T *s_res, c
   }
 }
 
-// acc += the slots [j0, j1) of an interface node, in slot order.  A node has
-// one slot per patch touching it (two on a face shared by two patches, four
-// on an edge, up to eight at a corner of a structured patch grid): the loads
-// of up to eight slots are issued together, then added one by one in slot
-// order; longer lists continue in steps of eight.
+// acc += the slots [j0, j1) of an interface node, in slot order, two loads
+// in flight per step.
 template <typename T, int DIM>
 __device__ __forceinline__ void add_slots(const T *partial, int j0, int j1, T *acc) {
-  for (int j = j0; j < j1; j += 8) {
-    const int n = j1 - j < 8 ? j1 - j : 8;
-    T u[8][DIM];
+  int j = j0;
+  for (; j + 2 <= j1; j += 2) {
+    T u[DIM], w[DIM];
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
-      if (r < n)
+    for (int c = 0; c < DIM; ++c) {
+      u[c] = partial[int64_t(j) * DIM + c];
+      w[c] = partial[int64_t(j + 1) * DIM + c];
+    }
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) u[r][c] = partial[int64_t(j + r) * DIM + c];
+    for (int c = 0; c < DIM; ++c) acc[c] = (acc[c] + u[c]) + w[c];
+  }
+  if (j < j1) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
-      if (r < n)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) acc[c] += u[r][c];
+    for (int c = 0; c < DIM; ++c) acc[c] += partial[int64_t(j) * DIM + c];
   }
 }
 
