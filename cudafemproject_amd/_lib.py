@@ -18,9 +18,10 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 OK = 0
 E_ARG, E_HIP, E_NONFINITE, E_MAXIT, E_BREAKDOWN, E_BAD_MESH, E_STATE, E_NOMEM, E_UNSUPPORTED = (
     -1, -2, -3, -4, -5, -6, -7, -8, -9)
+E_STAGNATION = -10
 STATUS_NAMES = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_NONFINITE", -4: "E_MAXIT",
                 -5: "E_BREAKDOWN", -6: "E_BAD_MESH", -7: "E_STATE", -8: "E_NOMEM",
-                -9: "E_UNSUPPORTED"}
+                -9: "E_UNSUPPORTED", -10: "E_STAGNATION"}
 
 FP64, FP32, MIXED = 0, 1, 2
 TOL_REF_GAMMA, TOL_REL_RES = 0, 1
@@ -340,14 +341,18 @@ class Context:
 
     def solve_rhs(self, f: np.ndarray, precision: int = FP64, tol: float = 1e-8,
                   tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
-                  check_every: int = 32, recurrence: int = CG_STANDARD):
-        """Solve K u = f for an explicit right-hand side (n_nodes x dim)."""
+                  check_every: int = 32, recurrence: int = CG_STANDARD,
+                  raise_on_error: bool = True):
+        """Solve K u = f for an explicit right-hand side (n_nodes x dim).
+        raise_on_error=False returns the solution and stats of a solve that
+        stopped with an error status (stats["status"])."""
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n_nodes, self.dim)
         o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every, recurrence)
         u = np.zeros((self.n_nodes, self.dim))
         s = Stats()
-        self._check(self.lib.femgpu_solve_rhs(self.h, C.byref(o), _dp(f), _dp(u), C.byref(s)),
-                    "solve_rhs")
+        st = self.lib.femgpu_solve_rhs(self.h, C.byref(o), _dp(f), _dp(u), C.byref(s))
+        if raise_on_error or st in (E_ARG, E_HIP, E_STATE, E_NOMEM):
+            self._check(st, "solve_rhs")
         return u, self._stats(s)
 
     def material_grad(self, u: np.ndarray, w: np.ndarray) -> np.ndarray:

@@ -146,7 +146,7 @@ struct RhsState {
   double gam[4], ff[4], thr[4], rr_last[4];
   double pq[4], g[4], rr[4], g_rep[4], rr_rep[4];
   double alpha[4], beta[4];
-  int active[4], status[4], iters[4], nrep[4], trig[4], restart[4];
+  int active[4], status[4], iters[4], nrep[4], trig[4], restart[4], stag[4];
   int any;
 };
 
@@ -202,6 +202,7 @@ __global__ __launch_bounds__(BT) void k_batch_pcg(BatchArgs<T> a) {
     S.rr_last[tid] = ff;
     S.iters[tid] = 0;
     S.nrep[tid] = 0;
+    S.stag[tid] = 0;
   }
   __syncthreads();
   if (tid == 0) {
@@ -294,6 +295,13 @@ __global__ __launch_bounds__(BT) void k_batch_pcg(BatchArgs<T> a) {
         } else if (rr <= S.thr[tid]) {
           S.active[tid] = 0;
         } else {
+          // two replacements in a row without a decrease of the FP64
+          // residual: the tolerance is below the attainable accuracy
+          S.stag[tid] = rr >= S.rr_last[tid] ? S.stag[tid] + 1 : 0;
+          if (S.stag[tid] >= 2) {
+            S.status[tid] = FEMGPU_E_STAGNATION;
+            S.active[tid] = 0;
+          }
           S.restart[tid] = rr > 0.25 * S.rr_last[tid];
           S.rr_last[tid] = rr;
           S.g[tid] = S.g_rep[tid];

@@ -48,7 +48,7 @@ struct BLState {
   int32_t it[2];
   int32_t state, status, iters, nrep;
   int32_t restart;     // set with state 2: restart the directions (p = z)
-  int32_t pad;
+  int32_t stag;        // consecutive replacements without a decrease of the FP64 residual
 };
 
 template <typename T>

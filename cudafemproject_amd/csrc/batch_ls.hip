@@ -172,8 +172,11 @@ __global__ __launch_bounds__(PT) void k_bl_patch(BLPatch<T> a) {
         }
         // Every patch of the instance reaches the same decision from the same
         // partials and state: the fields the first patch rewrites in this
-        // launch (state, thr) lead to the same outcome whether read before or
-        // after, and the others (rr_last, restart) change only in k_bl_mark.
+        // launch (state 0 -> 1 or 3, thr) lead to the same outcome whether
+        // read before or after, and the others (rr_last, restart) change
+        // only in k_bl_mark.  A replaced system (state 2) stays in state 2
+        // through this launch, since its restart decision is taken only in
+        // that state; the update launch that follows moves it to state 0.
         const bool act = !done && !pause;
         const double bt = act && !restart ? gv / S.g[q1] : 0.0;
         if (pid == I.p0) {
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(PT) void k_bl_patch(BLPatch<T> a) {
               S.it[1] = S.it[q1];
             }
           } else {
-            S.state = ns;
+            if (s != 2) S.state = ns;
             S.g[a.parity] = gv;
             S.b[a.parity] = bt;
             S.it[a.parity] = j;
@@ -395,7 +398,11 @@ __global__ __launch_bounds__(BL_UPD_NODES * NR) void k_bl_update(BLUpdate<T> a) 
     const int r = threadIdx.x;
     if (r < NR) {
       BLState &S = a.m.st[int64_t(B.inst) * NR + r];
-      int act = S.state == 0 || S.state == 2;  // 1: paused for a replacement, 3: finished
+      const int st0 = S.state;
+      int act = st0 == 0 || st0 == 2;  // 1: paused for a replacement, 3: finished
+      // a replaced system iterates again from here on (state 2 -> 0, set by
+      // the instance's first block; the others read 2 or 0, both active)
+      if (st0 == 2 && blockIdx.x == I.u0) S.state = 0;
       double al = 0.0;
       if (act) {
         al = S.g[a.parity] / pq[r];
@@ -573,9 +580,24 @@ __global__ __launch_bounds__(1024) void k_bl_mark(int n_inst, int nr, const BLIn
       // since the previous replacement, or when the recurrence had drifted
       // far from it (FP64 residual more than twice the recurrence's)
       S.restart = rr > 0.25 * S.rr_last || rr > 4.0 * S.rr ? 1 : 0;
+      // two replacements in a row without any decrease of the FP64 residual
+      // (the tolerance is below the attainable accuracy): stop the system
+      // here, where its residual is still near the best reached, instead of
+      // iterating on rounding noise
+      if (S.nrep >= 1 && rr >= S.rr_last) {
+        ++S.stag;
+      } else {
+        S.stag = 0;
+      }
       S.rr_last = rr;
-      S.state = 2;
       ++S.nrep;
+      if (S.stag >= 2 && !(rr <= S.thr)) {
+        S.state = 3;
+        S.status = FEMGPU_E_STAGNATION;
+        S.iters = S.it[1];
+      } else {
+        S.state = 2;
+      }
     }
     c += S.state != 3 ? 1.0 : 0.0;
   }

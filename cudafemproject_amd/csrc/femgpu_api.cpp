@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -436,23 +437,60 @@ int pcg_ws(femgpu_ctx *ctx, const char *tag, PcgWs<T> &w) {
   return FEMGPU_OK;
 }
 
+// One replay of a block of PCG iterations: a captured HIP graph, or, with
+// FEMGPU_NO_GRAPHS=1 in the environment (for tracing tools that cannot
+// follow graph launches), the same kernel launches issued directly.
+struct Replay {
+  hipGraphExec_t exec = nullptr;
+  std::function<void()> eager;
+  hipError_t launch(hipStream_t s) const {
+    if (exec) return hipGraphLaunch(exec, s);
+    eager();
+    return hipGetLastError();
+  }
+};
+
+bool graphs_disabled() {
+  static const bool off = [] {
+    const char *v = std::getenv("FEMGPU_NO_GRAPHS");
+    return v && *v && *v != '0';
+  }();
+  return off;
+}
+
 // Graph of `block` fused PCG iterations (block even, so every replay starts
 // at parity 0) on these buffers; captured once and cached.
 template <typename T>
 int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_tag,
-              CgScalars *sc, double *part_patch, double *part_node, hipGraphExec_t *out) {
+              CgScalars *sc, double *part_patch, double *part_node, Replay *out) {
+  if (graphs_disabled()) {
+    femgpu_ctx *c = ctx;
+    PcgWs<T> ws = w;
+    out->exec = nullptr;
+    out->eager = [=]() {
+      for (int k = 0; k < block; ++k)
+        femgpu::launch_pcg_iter<T>(c->dm, ws.x, ws.r, ws.z, ws.p, ws.q, ws.partial, ws.invD, sh,
+                                   sc, part_patch, part_node, k & 1, c->stream);
+    };
+    return FEMGPU_OK;
+  }
   if (ctx->graph_gen != ctx->buf_gen) {
     drop_graphs(ctx);
     ctx->graph_gen = ctx->buf_gen;
   }
-  char keybuf[256];
-  std::snprintf(keybuf, sizeof(keybuf), "%d/%d/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block,
-                (const void *)w.x, (const void *)w.invD, (const void *)sc, (const void *)sh.m,
-                (const void *)w.partial, (const void *)part_patch, double(sh.cK), double(sh.cM));
+  // the key names every pointer and value the captured launches hold
+  // (besides the mesh, whose change drops every graph)
+  char keybuf[320];
+  std::snprintf(keybuf, sizeof(keybuf),
+                "pcg/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block,
+                (const void *)w.x, (const void *)w.r, (const void *)w.z, (const void *)w.p,
+                (const void *)w.q, (const void *)w.partial, (const void *)w.invD, (const void *)sc,
+                (const void *)sh.m, (const void *)part_patch, (const void *)part_node,
+                double(sh.cK), double(sh.cM));
   const std::string key(keybuf);
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end()) {
-    *out = it->second;
+    out->exec = it->second;
     return FEMGPU_OK;
   }
   hipGraph_t g;
@@ -465,7 +503,7 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
   HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
   ctx->graphs[key] = exec;
-  *out = exec;
+  out->exec = exec;
   return FEMGPU_OK;
 }
 
@@ -531,8 +569,8 @@ struct ReplayPredict {
 // either advances the iteration counter or sets the stop flag; anything
 // else means the kernels did not run.  On return the stream is idle and
 // h_sc[0] holds the last scalars.
-int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block, int max_iter,
-                 double pause_host = 0.0, hipGraphExec_t tail = nullptr, int tail_block = 0,
+int replay_until(femgpu_ctx *ctx, const Replay &exec, CgScalars *sc, int block, int max_iter,
+                 double pause_host = 0.0, const Replay *tail = nullptr, int tail_block = 0,
                  ReplayPredict *pred = nullptr) {
   int st;
   if (!tail) tail_block = block;
@@ -541,7 +579,7 @@ int replay_until(femgpu_ctx *ctx, hipGraphExec_t exec, CgScalars *sc, int block,
   ReplayPredict &P = pred ? *pred : local;
   int n_slot[2] = {0, 0};  // iterations of the replay in each slot
   auto launch = [&](int slot, int n) -> int {
-    HIPCHK(hipGraphLaunch(n == block ? exec : tail, ctx->stream));
+    HIPCHK((n == block ? exec : *tail).launch(ctx->stream));
     HIPCHK(hipMemcpyAsync(&ctx->h_sc[slot], sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev_poll[slot], ctx->stream));
@@ -607,10 +645,10 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
   const int tb = tail_block_of(block);
-  hipGraphExec_t exec, tail;
+  Replay exec, tail;
   if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node, &exec)) ||
       (st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, tail, tb)))
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb)))
     return st;
   *iters = ctx->h_sc->iter_final;
   *done_code = ctx->h_sc->done;
@@ -641,18 +679,35 @@ int cg1_ws(femgpu_ctx *ctx, const char *tag, femgpu::Cg1State<T> &w) {
 // starts at parity 0); captured once per buffer set and cached.
 template <typename T>
 int cg1_graph(femgpu_ctx *ctx, const femgpu::Cg1State<T> &w, const T *invD, int block,
-              int prec_tag, CgScalars *sc, hipGraphExec_t *out) {
+              int prec_tag, CgScalars *sc, Replay *out) {
+  if (graphs_disabled()) {
+    femgpu_ctx *c = ctx;
+    const femgpu::Cg1State<T> ws = w;
+    out->exec = nullptr;
+    out->eager = [=]() {
+      for (int k = 0; k < block; ++k) femgpu::launch_cg1_iter<T>(c->dm, ws, invD, sc, k & 1, c->stream);
+    };
+    return FEMGPU_OK;
+  }
   if (ctx->graph_gen != ctx->buf_gen) {
     drop_graphs(ctx);
     ctx->graph_gen = ctx->buf_gen;
   }
-  char keybuf[160];
-  std::snprintf(keybuf, sizeof(keybuf), "cg1/%d/%d/%p/%p/%p/%p", prec_tag, block,
-                (const void *)w.x[0], (const void *)invD, (const void *)sc, (const void *)w.part);
-  const std::string key(keybuf);
+  // the key names every pointer the captured launches hold (besides the
+  // mesh, whose change drops every graph)
+  std::string key = "cg1/" + std::to_string(prec_tag) + "/" + std::to_string(block);
+  {
+    char b[32];
+    const void *ptrs[] = {w.x[0], w.x[1], w.r[0], w.r[1], w.p[0], w.p[1], w.s[0], w.s[1],
+                          w.w,    w.slot[0], w.slot[1], w.part, invD, sc};
+    for (const void *q : ptrs) {
+      std::snprintf(b, sizeof(b), "/%p", q);
+      key += b;
+    }
+  }
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end()) {
-    *out = it->second;
+    out->exec = it->second;
     return FEMGPU_OK;
   }
   hipGraph_t g;
@@ -664,7 +719,7 @@ int cg1_graph(femgpu_ctx *ctx, const femgpu::Cg1State<T> &w, const T *invD, int 
   HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
   ctx->graphs[key] = exec;
-  *out = exec;
+  out->exec = exec;
   return FEMGPU_OK;
 }
 
@@ -690,11 +745,11 @@ int run_cg1(femgpu_ctx *ctx, femgpu::Cg1State<T> &w, const T *invD, int kind, do
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
   const int tb = tail_block_of(block);
-  hipGraphExec_t exec, tail;
+  Replay exec, tail;
   int st;
   if ((st = cg1_graph<T>(ctx, w, invD, block, prec_tag, sc, &exec)) ||
       (st = cg1_graph<T>(ctx, w, invD, tb, prec_tag, sc, &tail)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, tail, tb)))
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb)))
     return st;
   *pos = cg1_pos(*ctx->h_sc);
   *done_code = ctx->h_sc->done;
@@ -1464,18 +1519,18 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
     if ((st = poll(ctx, sc))) return st;
     const double ff = ctx->h_sc->rr0;
     const int tb = tail_block_of(block_m);
-    hipGraphExec_t exec, tail;
+    Replay exec, tail;
     if ((st = cg1_graph<float>(ctx, w, invD, block_m, 3, sc, &exec)) ||
         (st = cg1_graph<float>(ctx, w, invD, tb, 3, sc, &tail)))
       return st;
-    int status = FEMGPU_OK, replacements = 0;
+    int status = FEMGPU_OK, replacements = 0, stag = 0;
     bool kept_p = false;  // the last replacement kept the search direction
     double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
     // The iterations pause on the device (done = 5) as soon as the FP32
     // residual has dropped by `drop`: run further, the FP32 recurrences of
     // s and w lose the accuracy the step-length formula relies on.
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0, tail, tb))) return st;
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0, &tail, tb))) return st;
       done = ctx->h_sc->done;
       pos = cg1_pos(*ctx->h_sc);
       if (trace_enabled())
@@ -1504,6 +1559,11 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
       if (rel <= o.tol) break;
       if (pos.iters >= max_iter || replacements > 1000) {
         status = FEMGPU_E_MAXIT;
+        break;
+      }
+      stag = replacements > 0 && rel >= rel_prev ? stag + 1 : 0;
+      if (stag >= 2) {
+        status = FEMGPU_E_STAGNATION;
         break;
       }
       // Restart (p = s = 0) when the FP64 residual did not at least halve
@@ -1632,7 +1692,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     if ((st = poll(ctx, sc))) return st;
     const double ff = ctx->h_sc->rr0;
     const int tb = tail_block_of(block_m);
-    hipGraphExec_t exec, tail;
+    Replay exec, tail;
     if ((st = pcg_graph<float>(ctx, w, OpShift<float>(), block_m, 1, sc, part_patch, part_node,
                                &exec)) ||
         (st = pcg_graph<float>(ctx, w, OpShift<float>(), tb, 1, sc, part_patch, part_node,
@@ -1641,7 +1701,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     // the stop prediction carries over replacements: same iteration
     // counter, the rate of the previous inner solve, the new residual
     ReplayPredict pred;
-    int status = FEMGPU_OK, replacements = 0;
+    int status = FEMGPU_OK, replacements = 0, stag = 0;
     bool kept_p = false;  // the last replacement kept the search direction
     double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
     // The host checks after every replay (of check_every iterations) whether
@@ -1650,7 +1710,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     // residual is formed after that one.
     double pause_rr = ff * drop * drop;
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause_rr, tail, tb, &pred)))
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, pause_rr, &tail, tb, &pred)))
         return st;
       const int done = ctx->h_sc->done;
       const double rr_rec = ctx->h_sc->rr_final;
@@ -1677,6 +1737,13 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
       if (rel <= o.tol) break;
       if (iters >= max_iter || replacements > 1000) {
         status = FEMGPU_E_MAXIT;
+        break;
+      }
+      // two replacements in a row without a decrease of the FP64 residual:
+      // the tolerance is below the attainable accuracy (stop near the best)
+      stag = replacements > 0 && rel >= rel_prev ? stag + 1 : 0;
+      if (stag >= 2) {
+        status = FEMGPU_E_STAGNATION;
         break;
       }
       // Reliable update: r is replaced by the FP64 residual, the directions

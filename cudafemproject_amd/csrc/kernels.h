@@ -104,6 +104,7 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial
                      const T *invD, OpShift<T> sh, CgScalars *sc, double *part_patch,
                      double *part_node, int parity, hipStream_t s);
 constexpr int NODE_PARTS = 512;  // blocks of the node-parallel PCG kernels
+static_assert(NODE_PARTS % 256 == 0, "the patch kernels read the partials in chunks of 256");
 
 // r = f - (interface-finished) K x, rr -> sc->rr (FP64 outer loop of MIXED).
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,

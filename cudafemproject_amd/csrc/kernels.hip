@@ -138,11 +138,36 @@ void k_ebe_patch(PatchArgs<T> a) {
     // Reduce the previous update's r'z and r'r (fixed order, every block
     // gets the same values), run the stop test, form beta.
     double g = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < NODE_PARTS; i += PT) {
-      g += a.prz[i];
-      q += a.prr[i];
+    if constexpr (sizeof(T) == 8) {
+      // FP64 (four workgroups per CU, two rounds of patches): every wave
+      // sums all partials itself, in the same order in every wave, with no
+      // block barrier (measured 0.8 us faster per launch at the 64^3 cube;
+      // no gain in FP32, where eight workgroups per CU hide the barrier)
+      const int lane = threadIdx.x & 63;
+      constexpr int NL = NODE_PARTS / 64;
+#pragma unroll
+      for (int c0 = 0; c0 < NL; c0 += 4) {
+        double gv[4], qv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          gv[j] = a.prz[lane + 64 * (c0 + j)];
+          qv[j] = a.prr[lane + 64 * (c0 + j)];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          g += gv[j];
+          q += qv[j];
+        }
+      }
+      g = wave_sum(g);
+      q = wave_sum(q);
+    } else {
+      for (int i = threadIdx.x; i < NODE_PARTS; i += PT) {
+        g += a.prz[i];
+        q += a.prr[i];
+      }
+      block_sum2<PT>(g, q, s_red);
     }
-    block_sum2<PT>(g, q, s_red);
     const int q1 = a.parity ^ 1;
     const int n_upd = sc->it[q1] + 1;
     int stop = 0;

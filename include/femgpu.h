@@ -41,7 +41,11 @@ enum {
   FEMGPU_E_BAD_MESH = -6,   /* degenerate element, bad index, face not on elem */
   FEMGPU_E_STATE = -7,      /* call order (e.g. solve before set_mesh)       */
   FEMGPU_E_NOMEM = -8,      /* device allocation failed                      */
-  FEMGPU_E_UNSUPPORTED = -9 /* input the reference only handles by accident  */
+  FEMGPU_E_UNSUPPORTED = -9,/* input the reference only handles by accident  */
+  FEMGPU_E_STAGNATION = -10 /* MIXED / batch: the FP64 residual stopped
+                               decreasing above the tolerance (two residual
+                               replacements in a row without progress: the
+                               tolerance is below the attainable accuracy)  */
 };
 
 /* Arithmetic of a solve. */

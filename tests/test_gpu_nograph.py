@@ -1,0 +1,54 @@
+"""GPU: FEMGPU_NO_GRAPHS=1 (PCG iteration blocks launched directly instead of
+replayed as captured HIP graphs, for tracing tools) runs the same kernels in
+the same order: bitwise the same displacements and iteration counts, for
+both PCG recurrences and the mixed solve."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cudafemproject_amd as F
+from conftest import ROOT
+from cudafemproject_amd.problem import cube_problem
+
+pytestmark = pytest.mark.gpu
+
+CHILD = r'''
+import json, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import cudafemproject_amd as F
+from cudafemproject_amd.problem import cube_problem
+pb = cube_problem(10, convention=F.LOADS_REFERENCE)
+out = {}
+with F.Context(0) as c:
+    c.set_tuning(200, 512)
+    c.set_problem(pb)
+    for name, prec, rec in (("fp64", F.FP64, F.CG_STANDARD), ("cg1", F.FP64, F.CG_SINGLE_REDUCTION),
+                            ("mixed", F.MIXED, F.CG_STANDARD)):
+        u, s = c.solve(prec, tol=1e-10, tol_kind=F.TOL_REL_RES, recurrence=rec)
+        out[name] = [u.ravel().tolist(), s["iterations"]]
+print(json.dumps(out))
+'''
+
+
+def _run(no_graphs):
+    env = dict(os.environ)
+    env.pop("FEMGPU_NO_GRAPHS", None)
+    if no_graphs:
+        env["FEMGPU_NO_GRAPHS"] = "1"
+    r = subprocess.run([sys.executable, "-c", CHILD, ROOT], capture_output=True, text=True,
+                       env=env, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_no_graphs_matches_graphs():
+    a, b = _run(False), _run(True)
+    for k in a:
+        print(f"[nograph] {k}: iterations {a[k][1]} / {b[k][1]}", flush=True)
+        assert a[k][1] == b[k][1], k
+        assert np.array_equal(np.array(a[k][0]), np.array(b[k][0])), k

@@ -200,3 +200,28 @@ def test_replay_length_does_not_change_the_solve(ctx, rec):
             else:
                 assert s["iterations"] == ref[1], (prec, ce)
                 assert np.array_equal(u, ref[0]), (prec, ce)
+
+
+def test_mixed_below_attainable_accuracy_stagnates():
+    """A tolerance below what FP64 PCG attains on an ill-conditioned plate
+    (instance 21 of the batched workload, body-force load case): the mixed
+    solve stops with E_STAGNATION after two residual replacements without
+    progress, with its residual at the floor, instead of iterating on."""
+    from cudafemproject_amd.problem import plate_batch
+    b = plate_batch(22, seed=2024)[21]
+    c = F.Context(0)
+    try:
+        c.set_mesh(b.nodes, b.elems)
+        c.set_material(b.E, b.nu)
+        c.set_constraints(b.constr_node, b.constr_code)
+        f = b.f[:, 3, :]
+        u64, s64 = c.solve_rhs(f, F.FP64, tol=1e-8)
+        opts = dict(tol=1e-13, max_iter=50 * s64["iterations"])
+        for rec in (F.CG_STANDARD, F.CG_SINGLE_REDUCTION):
+            um, sm = c.solve_rhs(f, F.MIXED, recurrence=rec, raise_on_error=False, **opts)
+            print(f"[stagnation] recurrence {rec}: {sm}", flush=True)
+            assert sm["status"] == -10
+            assert sm["rel_res"] <= 1e-8 and sm["iterations"] < opts["max_iter"]
+            assert rel(um, u64) < 1e-4
+    finally:
+        c.close()
