@@ -545,6 +545,8 @@ def main():
                    "patch_tuning": {"threads": 256, "elements": 768}},
         "fp64_time_to_solve_s": float(t64),
         "speedup_mixed_vs_fp64": float(t64_best / tmx),
+        # the same (standard) recurrence in both precisions
+        "speedup_mixed_vs_fp64_standard_recurrence": float(t64 / tmx),
         "fp64_single_reduction_time_to_solve_s": t64s,
         "mixed_single_reduction_time_to_solve_s": tmxs,
         "iterations_single_reduction_fp64": it64s,

@@ -52,8 +52,8 @@ __host__ __device__ inline BLLds bl_lds(int dim, int nv, int tb, int pn, int ent
 }
 
 // Sum of vals[i * NV + v] over i in [i0, i1) for every v, by the 64 lanes of
-// one wave (fixed order: strided lanes, then the xor butterfly, which gives
-// every lane the same bits).  Call from one full wave.
+// one wave (fixed order: strided lanes, then wave_sum, which gives every
+// lane the same bits).  Call from one full wave.
 template <int NV>
 __device__ __forceinline__ void wave_sums(const double *vals, int i0, int i1, double *out) {
   const int lane = threadIdx.x & 63;

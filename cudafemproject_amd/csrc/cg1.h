@@ -7,6 +7,9 @@
 
 namespace femgpu {
 
+// Per-patch partial sums of one single-reduction launch.
+constexpr int CG1_PARTS = 5;
+
 // Device state of the single-reduction PCG.  [2]: ping-pong by launch
 // parity; the first iteration launch (parity 0) reads index 0.
 template <typename T>
@@ -14,33 +17,29 @@ struct Cg1State {
   T *x[2], *r[2], *p[2], *s[2];  // n_nodes x dim each
   T *w;                          // w = A u, owned nodes (interface nodes: slots)
   T *slot[2];                    // interface partial slots of w (n_slots x dim)
-  double *part;                  // [2][3][np]: per-patch r'u, u'A u, r'r
+  // [2][CG1_PARTS][np]: per-patch r'u, u'A u, r'r, u's_prev, p_prev's_prev
+  // (the last two in FP32 only)
+  double *part;
 };
 
 hipError_t prepare_cg1_kernels();
 
 // w = A u with u = M^-1 r[0] (owned nodes into w, interface nodes into
-// slot[0]) and the partials part[0]; reads nothing else of the state.
+// slot[0]) and the partials part[0]; reads r[0] (FP32 also p[0] and s[0]).
 template <typename T>
 void launch_cg1_init(const DevMesh &dm, const Cg1State<T> &st, const T *invD, hipStream_t s);
 // One iteration: reads index `parity` of the state, writes the other.
 template <typename T>
 void launch_cg1_iter(const DevMesh &dm, const Cg1State<T> &st, const T *invD, CgScalars *sc,
                      int parity, hipStream_t s);
-// Partials of s'u and p's (u = M^-1 r) over n values into dots
-// (CG1_DOT_PARTS doubles), for launch_cg1_fin mode 1.
-constexpr int CG1_DOT_PARTS = 2 * 512;
-template <typename T>
-void launch_cg1_dots(int64_t n, const T *r, const T *p, const T *s_, const T *invD, double *dots,
-                     hipStream_t s);
 // Scalars from part[0] after launch_cg1_init.  mode 0: fresh start
 // (g0, thresholds from sc->kind / sc->eps, counters); 1: residual replacement
-// that keeps the search direction p[0] (s[0] = A p[0]; dots from
-// launch_cg1_dots on r[0], p[0], s[0]), the previous scalars taken from
-// parity q; 2: replacement with a restart (b = 0 at the next iteration).
-// pause_drop > 0: the iterations stop with done = 5 once the residual norm
-// has dropped by that factor (the next residual replacement).
+// that keeps the search direction p[0] (s[0] = A p[0]; FP32 state only), the
+// previous scalars taken from parity q; 2: replacement with a restart (b = 0
+// at the next iteration).  pause_drop > 0: the iterations stop with done = 5
+// once the residual norm has dropped by that factor (the next residual
+// replacement).
 void launch_cg1_fin(CgScalars *sc, const double *part, int np, int mode, int q,
-                    const double *dots, double pause_drop, hipStream_t s);
+                    double pause_drop, hipStream_t s);
 
 }  // namespace femgpu

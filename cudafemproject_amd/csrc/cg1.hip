@@ -12,12 +12,20 @@
 //   x_{j+1} = x_j + a_j p_j,  r_{j+1} = r_j - a_j s_j,  u_{j+1} = M^-1 r_{j+1},
 //   w_{j+1} = A u_{j+1},  g_{j+1} = r_{j+1}'u_{j+1},  d_{j+1} = w_{j+1}'u_{j+1},
 //   b_{j+1} = g_{j+1} / g_j,  a_{j+1} = g_{j+1} / (d_{j+1} - b_{j+1} g_{j+1} / a_j).
-// In exact arithmetic the iterates equal the reference's.
+// In exact arithmetic the iterates equal the reference's.  That step length
+// rests on the orthogonality of successive residuals, which FP32 vectors lose
+// within a few hundred iterations (the mixed solve needed 1102 iterations at
+// the 64^3 cube against the standard recurrence's 870).  The FP32 kernel
+// therefore forms the denominator p_{j+1}'A p_{j+1} itself, over terms the same
+// reduction delivers: d_{j+1} + 2 b_{j+1} e_{j+1} + b_{j+1}^2 z_{j+1} with
+// e_{j+1} = u_{j+1}'s_j and z_{j+1} = p_j's_j (s_j'u_{j+1} = p_j'w_{j+1} by the
+// symmetry of A), which assumes only s = A p (938 iterations there).  FP64
+// keeps the paper's form, which matches the standard recurrence's count.
 //
 // One launch = one iteration, one workgroup per patch (patch_common.h):
 //  * preamble: every block reduces the previous launch's per-patch partials
-//    of g, d and r'r in the same fixed order, runs the stop test and forms
-//    a_j, b_j;
+//    of g, d and r'r (FP32 also e and z) in the same fixed order, runs the
+//    stop test and forms a_j, b_j;
 //  * phase A, one patch node per thread: w_j of the node (owned: the vector;
 //    interface: the fixed-order sum of its slots), then s_j, r_{j+1},
 //    u_{j+1}.  Every patch that touches an interface node computes these
@@ -25,8 +33,8 @@
 //    patch also forms p_j, x_{j+1} and writes the node's state;
 //  * phases B and C: w_{j+1} = A u_{j+1} (owned nodes in place, interface
 //    nodes into slots) and the patch's partial of d_{j+1} = sum_e u_e'K_e u_e
-//    (+ constrained diagonal terms); g and r'r partials over the nodes the
-//    patch writes.
+//    (+ constrained diagonal terms); g and r'r (FP32: e, z) partials over the
+//    nodes the patch writes.
 // The node state (x, r, p, s) and the slots ping-pong by the launch's parity,
 // so no block reads a value another block of the same launch writes.  The
 // parity follows the launches (every graph replay starts at parity 0), not
@@ -62,7 +70,9 @@ struct Cg1Args {
 };
 
 // INIT: w = A u, u = M^-1 r[0], into w / slot[0], and the partials of g, d,
-// r'r into part[0] (the state the first iteration launch reads).
+// r'r (FP32: e = u's[0], z = p[0]'s[0]) into part[0], the state the first
+// iteration launch reads (p[0] = s[0] = 0 at a start or restart; the kept
+// direction and s[0] = A p[0] after a residual replacement).
 template <typename T, int DIM, int PT, bool INIT>
 __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   constexpr unsigned FMASK = (1u << DIM) - 1u;
@@ -71,7 +81,7 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   // XCD-aware: consecutive patches share interface nodes and an L2
   const int pid = (b & 7) * a.p8 + (b >> 3);
   if (pid >= a.np) return;
-  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.ent);
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), a.pn, a.ent, RED_CG1);
   T *s_c = reinterpret_cast<T *>(lds + L.off_c);
   T *s_x = reinterpret_cast<T *>(lds + L.off_x);      // u, constrained components zeroed
   T *s_res = reinterpret_cast<T *>(lds + L.off_res);  // one component of the entries
@@ -79,19 +89,25 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   const int in = INIT ? 0 : a.parity;
   const int out = INIT ? 0 : a.parity ^ 1;
   const Cg1State<T> &S = a.st;
+  // FP32: step length from p'A p itself (e and z partials); FP64: the
+  // paper's recurrence
+  constexpr bool EXACT = sizeof(T) == 4;
+  constexpr int NPART = EXACT ? 5 : 3;
 
   T alpha = T(0), beta = T(0);
   if constexpr (!INIT) {
     CgScalars *sc = a.sc;
     if (sc->done) return;
-    const double *pin = S.part + size_t(in) * 3 * a.np;
-    double g = 0.0, d = 0.0, q = 0.0;
+    const double *pin = S.part + size_t(in) * CG1_PARTS * a.np;
+    double v[NPART];
+#pragma unroll
+    for (int m = 0; m < NPART; ++m) v[m] = 0.0;
     for (int i = threadIdx.x; i < a.np; i += PT) {
-      g += pin[i];
-      d += pin[a.np + i];
-      q += pin[2 * a.np + i];
+#pragma unroll
+      for (int m = 0; m < NPART; ++m) v[m] += pin[m * a.np + i];
     }
-    block_sum3<PT>(g, d, q, s_red);
+    block_sum_n<PT, NPART>(v, s_red);
+    const double g = v[0], d = v[1], q = v[2];
     const int q1 = in ^ 1;
     const int j = sc->it[q1] + 1;
     int stop = 0;
@@ -108,7 +124,11 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
     if (!stop) {
       // b_j and a_j; g[q1] = a[q1] = +inf at a (re)start gives b = 0, a = g/d
       bt = g / sc->g[q1];
-      const double den = d - bt * g / sc->al[q1];
+      double den;
+      if constexpr (EXACT)
+        den = d + bt * (2.0 * v[3] + bt * v[4]);
+      else
+        den = d - bt * g / sc->al[q1];
       al = g / den;
       if (!(den > 0.0) || !isfinite(al)) stop = isfinite(den) ? 3 : 2;
     }
@@ -160,7 +180,7 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
   constexpr bool REREAD = sizeof(T) == 4;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) u[c] = T(0);
-  double acc_g = 0.0, acc_d = 0.0, acc_q = 0.0;
+  double acc_g = 0.0, acc_d = 0.0, acc_q = 0.0, acc_e = 0.0, acc_z = 0.0;
   if (has_node) {
     int s0 = 0, s1 = 0;
     if (own) {
@@ -185,6 +205,14 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
       for (int c = 0; c < DIM; ++c) {
         rn[c] = S.r[0][o + c];
         u[c] = a.invD[o + c] * rn[c];
+      }
+      if (EXACT && first) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          const T sv = S.s[0][o + c];
+          acc_e += double(u[c]) * double(sv);
+          acc_z += double(S.p[0][o + c]) * double(sv);
+        }
       }
     } else {
       T rv[DIM], sv[DIM], dv[DIM], wv[DIM], pv[DIM], xv[DIM];
@@ -223,6 +251,10 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
           S.s[out][o + c] = sn;
           S.x[out][o + c] = xv[c] + alpha * pn;
           S.r[out][o + c] = rn[c];
+          if (EXACT) {
+            acc_e += double(u[c]) * double(sn);
+            acc_z += double(pn) * double(sn);
+          }
         }
       }
     }
@@ -235,6 +267,12 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
     }
 #pragma unroll
     for (int c = 0; c < DIM; ++c) s_x[k * DIM + c] = ((fm >> c) & 1u) ? T(0) : u[c];
+  }
+  // wave sums of e and z now: wave-uniform, they stay in scalar registers
+  // through the assembly
+  if constexpr (EXACT) {
+    acc_e = wave_sum(acc_e);
+    acc_z = wave_sum(acc_z);
   }
   __syncthreads();
 
@@ -280,49 +318,30 @@ __global__ __launch_bounds__(PT) void k_cg1(Cg1Args<T> a) {
       for (int c = 0; c < DIM; ++c) S.slot[out][slot * DIM + c] = acc[c];
     }
   }
-  block_sum3<PT>(acc_g, acc_d, acc_q, s_red);
-  if (threadIdx.x == 0) {
-    double *po = S.part + size_t(out) * 3 * a.np;
-    po[pid] = acc_g;
-    po[a.np + pid] = acc_d;
-    po[2 * a.np + pid] = acc_q;
+  double v[NPART];
+  v[0] = wave_sum(acc_g);
+  v[1] = wave_sum(acc_d);
+  v[2] = wave_sum(acc_q);
+  if constexpr (EXACT) {
+    v[3] = acc_e;
+    v[4] = acc_z;
   }
-}
-
-constexpr int DOT_BLOCKS = 512, DOT_THREADS = 256;
-
-// Partials of s'u and p's with u = M^-1 r (u formed as in k_cg1), for the
-// exact step length after a residual replacement.
-template <typename T>
-__global__ __launch_bounds__(DOT_THREADS) void k_cg1_dots(int64_t n, const T *r, const T *p,
-                                                          const T *s, const T *invD,
-                                                          double *out) {
-  __shared__ double s_red[2 * DOT_THREADS / 64];
-  double su = 0.0, ps = 0.0;
-  const int64_t stride = int64_t(gridDim.x) * DOT_THREADS;
-  for (int64_t i = int64_t(blockIdx.x) * DOT_THREADS + threadIdx.x; i < n; i += stride) {
-    const T u = invD[i] * r[i];
-    su += double(s[i]) * double(u);
-    ps += double(p[i]) * double(s[i]);
-  }
-  block_sum2<DOT_THREADS>(su, ps, s_red);
+  waves_sum_n<PT, NPART>(v, s_red);
   if (threadIdx.x == 0) {
-    out[blockIdx.x] = su;
-    out[DOT_BLOCKS + blockIdx.x] = ps;
+    double *po = S.part + size_t(out) * CG1_PARTS * a.np;
+#pragma unroll
+    for (int m = 0; m < NPART; ++m) po[m * a.np + pid] = v[m];
   }
 }
 
 // Scalars after an INIT launch.  mode 0: fresh start (g0, thresholds,
 // counters); 1: residual replacement keeping the search direction p, state
-// moved from parity q; 2: replacement with a restart (b = 0 next), also
-// after a breakdown stop (done = 3).
-// Mode 1 sets the stored previous step length so that the next launch's
-// denominator d - b g / a_prev equals p'A p of the new direction
-// p = u + b p_prev exactly: p'A p = d + 2 b s_prev'u + b^2 p_prev's_prev
-// (dots: the k_cg1_dots partials), instead of relying on the recurrence
-// identities that the replaced residual no longer satisfies.
+// moved from parity q (FP32 state only: its step length is p'A p of the new
+// direction p = u + b p_prev, exact for a replaced residual too, from the
+// INIT partials e = u's_prev and z = p_prev's_prev); 2: replacement with a
+// restart (b = 0 next), also after a breakdown stop (done = 3).
 __global__ void k_cg1_fin(CgScalars *sc, const double *part, int np, int mode, int q,
-                          const double *dots, double pause_drop) {
+                          double pause_drop) {
   __shared__ double s_red[3 * 256 / 64];
   double g = 0.0, d = 0.0, rr = 0.0;
   for (int i = threadIdx.x; i < np; i += 256) {
@@ -331,14 +350,6 @@ __global__ void k_cg1_fin(CgScalars *sc, const double *part, int np, int mode, i
     rr += part[2 * np + i];
   }
   block_sum3<256>(g, d, rr, s_red);
-  double su = 0.0, ps = 0.0;
-  if (mode == 1) {
-    for (int i = threadIdx.x; i < DOT_BLOCKS; i += 256) {
-      su += dots[i];
-      ps += dots[DOT_BLOCKS + i];
-    }
-    block_sum2<256>(su, ps, s_red);
-  }
   if (threadIdx.x != 0) return;
   const double inf = INFINITY;
   sc->g_final = g;
@@ -363,13 +374,7 @@ __global__ void k_cg1_fin(CgScalars *sc, const double *part, int np, int mode, i
     sc->b[1] = sc->b[0];
     sc->it[1] = sc->it[0];
   }
-  if (mode == 2) {
-    sc->g[1] = sc->al[1] = inf;
-  } else {
-    const double bt = g / sc->g[1];
-    const double pap = d + 2.0 * bt * su + bt * bt * ps;
-    sc->al[1] = (bt == 0.0 || pap == d) ? inf : bt * g / (d - pap);
-  }
+  if (mode == 2) sc->g[1] = sc->al[1] = inf;
   // a restart also clears a breakdown stop (the host restarts after one)
   if (sc->done == 1 || sc->done == 5 || (mode == 2 && sc->done == 3)) sc->done = 0;
 }
@@ -407,7 +412,7 @@ Cg1Args<T> cg1_args(const DevMesh &dm, const Cg1State<T> &st, const T *invD, CgS
 
 template <typename T, int DIM, int PT, bool INIT>
 void launch_pt(const DevMesh &dm, const Cg1Args<T> &a, hipStream_t s) {
-  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.ent);
+  const LdsLayout L = lds_layout(DIM, int(sizeof(T)), dm.pn, dm.ent, RED_CG1);
   hipLaunchKernelGGL((k_cg1<T, DIM, PT, INIT>), dim3(8 * a.p8), dim3(PT), L.total, s, a);
 }
 
@@ -466,21 +471,10 @@ void launch_cg1_iter(const DevMesh &dm, const Cg1State<T> &st, const T *invD, Cg
   launch<T, false>(dm, cg1_args<T>(dm, st, invD, sc, parity), s);
 }
 
-template <typename T>
-void launch_cg1_dots(int64_t n, const T *r, const T *p, const T *s_, const T *invD, double *dots,
-                     hipStream_t s) {
-  hipLaunchKernelGGL((k_cg1_dots<T>), dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, s, n, r, p, s_, invD,
-                     dots);
-}
-
 void launch_cg1_fin(CgScalars *sc, const double *part, int np, int mode, int q,
-                    const double *dots, double pause_drop, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg1_fin, dim3(1), dim3(256), 0, s, sc, part, np, mode, q, dots,
-                     pause_drop);
+                    double pause_drop, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg1_fin, dim3(1), dim3(256), 0, s, sc, part, np, mode, q, pause_drop);
 }
-
-template void launch_cg1_dots<float>(int64_t, const float *, const float *, const float *,
-                                     const float *, double *, hipStream_t);
 
 template void launch_cg1_init<double>(const DevMesh &, const Cg1State<double> &, const double *,
                                       hipStream_t);

@@ -18,10 +18,33 @@
 namespace femgpu {
 namespace elem {
 
+// v from another lane by a DPP pattern (both 32-bit halves); rows outside
+// ROW_MASK read 0.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// Sum over the 64 lanes of a full wave, the same bits in every lane (fixed
+// order).  DPP steps inside the VALU: lane pairs and quads (quad_perm), 8 and
+// 16 lanes (half-row and row mirrors), then rows 1 and 3 add the last lane of
+// the row before them (row_bcast:15) and rows 2 and 3 add lane 31
+// (row_bcast:31), which leaves the total in lane 63, broadcast by readlane.
+// No LDS traffic and no lane-address registers, unlike a shuffle butterfly.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1>(v);         // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);         // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);        // row_half_mirror
+  v += dpp_f64<0x140>(v);        // row_mirror
+  v += dpp_f64<0x142, 0xA>(v);   // row_bcast:15
+  v += dpp_f64<0x143, 0xC>(v);   // row_bcast:31
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(b), 63);
+  const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(b >> 32), 63);
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
 // Block-wide sum, same value returned in every thread; fixed order.
@@ -84,6 +107,35 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
   a = ra;
   b = rb;
   c = rc;
+}
+
+// Block-wide sums of N values that are already wave-uniform (each wave's
+// total in every lane): one pair of barriers, sh holds N * BT / 64 doubles,
+// the totals in every thread, summed over the waves in order.
+template <int BT, int N>
+__device__ __forceinline__ void waves_sum_n(double (&v)[N], double *sh) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) sh[N * w + i] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < BT / 64; ++k) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += sh[N * k + i];
+  }
+}
+
+// N block-wide sums with one pair of barriers; sh holds N * BT / 64 doubles.
+template <int BT, int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double *sh) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+  waves_sum_n<BT, N>(v, sh);
 }
 
 // ---------------------------------------------------------------------------

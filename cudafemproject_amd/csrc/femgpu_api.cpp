@@ -670,7 +670,8 @@ int cg1_ws(femgpu_ctx *ctx, const char *tag, femgpu::Cg1State<T> &w) {
       return st;
   }
   if ((st = dbuf(ctx, t + "w", nd, &w.w)) ||
-      (st = dbuf(ctx, "cg1part", size_t(6) * std::max(ctx->dm.np, 1), &w.part)))
+      (st = dbuf(ctx, "cg1part", size_t(2 * femgpu::CG1_PARTS) * std::max(ctx->dm.np, 1),
+                 &w.part)))
     return st;
   return FEMGPU_OK;
 }
@@ -741,7 +742,7 @@ int run_cg1(femgpu_ctx *ctx, femgpu::Cg1State<T> &w, const T *invD, int kind, do
   femgpu::launch_fill<T>(w.s[0], nd, T(0), ctx->stream);
   femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
   femgpu::launch_cg1_init<T>(ctx->dm, w, invD, ctx->stream);
-  femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, nullptr, 0.0, ctx->stream);
+  femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, 0.0, ctx->stream);
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
   const int tb = tail_block_of(block);
@@ -1497,9 +1498,8 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
     femgpu::launch_convert<float, double>(w.x[pos.q], x64, nd, ctx->stream);
   } else if (o.precision == FEMGPU_MIXED) {
     femgpu::Cg1State<float> w;
-    double *r64, *y64, *part64, *part_node, *dots;
+    double *r64, *y64, *part64, *part_node;
     if ((st = cg1_ws(ctx, "32", w)) || (st = dbuf(ctx, "ir_r", size_t(nd), &r64)) ||
-        (st = dbuf(ctx, "cg1dots", size_t(femgpu::CG1_DOT_PARTS), &dots)) ||
         (st = dbuf(ctx, "ir_y", size_t(nd), &y64)) ||
         (st = dbuf(ctx, "partial64", size_t(ctx->plan.n_slots) * ctx->dim + 1, &part64)) ||
         (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
@@ -1514,7 +1514,7 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
     femgpu::launch_fill<float>(w.s[0], nd, 0.f, ctx->stream);
     femgpu::launch_reset_scalars(sc, FEMGPU_TOL_REL_RES, o.tol, max_iter, ctx->stream);
     femgpu::launch_cg1_init<float>(ctx->dm, w, invD, ctx->stream);
-    femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, nullptr, drop, ctx->stream);
+    femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, 0, 0, drop, ctx->stream);
     HIPCHK(hipGetLastError());
     if ((st = poll(ctx, sc))) return st;
     const double ff = ctx->h_sc->rr0;
@@ -1523,14 +1523,18 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
     if ((st = cg1_graph<float>(ctx, w, invD, block_m, 3, sc, &exec)) ||
         (st = cg1_graph<float>(ctx, w, invD, tb, 3, sc, &tail)))
       return st;
+    // the stop prediction carries over replacements (same iteration
+    // counter, the previous rate, the new residual)
+    ReplayPredict pred;
     int status = FEMGPU_OK, replacements = 0, stag = 0;
     bool kept_p = false;  // the last replacement kept the search direction
     double rel = ff > 0 ? 1.0 : 0.0, rel_prev = rel;
     // The iterations pause on the device (done = 5) as soon as the FP32
     // residual has dropped by `drop`: run further, the FP32 recurrences of
-    // s and w lose the accuracy the step-length formula relies on.
+    // s and w drift from A p and A u.
     while (rel > o.tol) {
-      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0, &tail, tb))) return st;
+      if ((st = replay_until(ctx, exec, sc, block_m, max_iter, 0.0, &tail, tb, &pred)))
+        return st;
       done = ctx->h_sc->done;
       pos = cg1_pos(*ctx->h_sc);
       if (trace_enabled())
@@ -1552,7 +1556,8 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
       femgpu::launch_residual64(ctx->dm, f64, y64, part64, r64, sc2, part_node, ctx->stream);
       HIPCHK(hipGetLastError());
       if ((st = poll(ctx, sc2))) return st;
-      rel = ff > 0 ? std::sqrt(ctx->h_sc->rr / ff) : 0.0;
+      const double rr64 = ctx->h_sc->rr;
+      rel = ff > 0 ? std::sqrt(rr64 / ff) : 0.0;
       if (trace_enabled())
         std::fprintf(stderr, "femgpu mixed/single-reduction: %d iterations, stop %d, FP64 rel-res %.3e\n",
                      pos.iters, done, rel);
@@ -1581,11 +1586,7 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
       }
       femgpu::launch_convert<double, float>(r64, w.r[0], nd, ctx->stream);
       femgpu::launch_cg1_init<float>(ctx->dm, w, invD, ctx->stream);
-      if (!restart)
-        femgpu::launch_cg1_dots<float>(ctx->dm.n_active * ctx->dim, w.r[0], w.p[0], w.s[0], invD,
-                                       dots, ctx->stream);
-      femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, restart ? 2 : 1, pos.q, dots, drop,
-                             ctx->stream);
+      femgpu::launch_cg1_fin(sc, w.part, ctx->dm.np, restart ? 2 : 1, pos.q, drop, ctx->stream);
       HIPCHK(hipGetLastError());
       if (trace_enabled()) {
         if ((st = poll(ctx, sc))) return st;
@@ -1594,6 +1595,7 @@ static int solve_cg1(femgpu_ctx *ctx, const femgpu_solve_opts &o, int max_iter, 
                      ctx->h_sc->done, ctx->h_sc->it[0], ctx->h_sc->it[1]);
       }
       rel_prev = rel;
+      pred.seed(pos.iters, rr64, rr64);
       ++replacements;
     }
     S->iterations = pos.iters;

@@ -1107,8 +1107,10 @@ hipError_t prepare_kernels() {
 
 int grid_blocks(int64_t n) { return grid_for(n); }
 
+// the largest of the patch kernels' layouts (the single-reduction kernel's
+// reduction area)
 int lds_bytes(int dim, int elem_bytes, int pn, int ent) {
-  return lds_layout(dim, elem_bytes, pn, ent).total;
+  return lds_layout(dim, elem_bytes, pn, ent, RED_CG1).total;
 }
 
 template <typename T>

@@ -24,16 +24,20 @@ struct LdsLayout {
 
 __host__ __device__ inline int al32(int v) { return (v + 31) & ~31; }  // 32-byte aligned regions
 
+// Reduction area of the single-reduction PCG kernel (cg1.hip): five block
+// sums of up to 1024 threads, the largest of the patch kernels'.
+constexpr int RED_CG1 = 5 * 16;
+
 // Patch kernel LDS: node coordinates and vector (pn x dim), one component of
-// the node-sorted element entries (ent, padded runs), reduction scratch for
-// up to three block sums of 1024 threads (48 doubles).
-__host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ent) {
+// the node-sorted element entries (ent, padded runs), reduction scratch of
+// red doubles (48: up to three block sums of 1024 threads).
+__host__ __device__ inline LdsLayout lds_layout(int dim, int tb, int pn, int ent, int red = 48) {
   LdsLayout L;
   L.off_c = 0;
   L.off_x = al32(pn * dim * tb);
   L.off_res = L.off_x + al32(pn * dim * tb);
   L.off_red = L.off_res + al32(ent * tb);
-  L.total = L.off_red + 48 * 8;
+  L.total = L.off_red + red * 8;
   return L;
 }
 
