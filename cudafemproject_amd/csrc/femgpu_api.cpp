@@ -897,6 +897,16 @@ int upload_plan(femgpu_ctx *ctx) {
     dm.sh_info = reinterpret_cast<const int4 *>(pi);
   }
   dm.node_end = ne_;
+  {
+    std::vector<int32_t> pnode;
+    std::vector<uint16_t> pend;
+    femgpu::pack_patch_nodes(P, threads, pnode, pend);
+    int32_t *pn_;
+    uint16_t *pe_;
+    if ((st = upload(ctx, "pnode", pnode, &pn_)) || (st = upload(ctx, "pend", pend, &pe_))) return st;
+    dm.pnode = reinterpret_cast<const int2 *>(pn_);
+    dm.pend = pe_;
+  }
   dm.slot_start = ss;
   dm.elems_int = ei;
   std::vector<double> c64(size_t(n_nodes) * dim);

@@ -82,4 +82,14 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
 //   w2 = p1 >> 10 | p2 << 4 | p3 << 18
 void pack_records(const MeshPlan &plan, std::vector<uint32_t> &rec);
 
+// Per-patch node tables at a fixed stride (entry p * stride + k for node k
+// of patch p, stride >= the patch's node count), so that a workgroup reads
+// them without first reading its PatchInfo: pnode holds 2 x int32 per
+// entry, the internal node id and, for an interface node, its slot with
+// bit 31 set when the slot is the node's first (0 for owned nodes); pend the
+// end of the node's run in the patch's entry list (node_end).  Entries
+// beyond a patch's nodes are zero.
+void pack_patch_nodes(const MeshPlan &plan, int stride, std::vector<int32_t> &pnode,
+                      std::vector<uint16_t> &pend);
+
 }  // namespace femgpu

@@ -24,6 +24,10 @@ struct DevMesh {
   const int32_t *sh_slot = nullptr;
   const int4 *sh_info = nullptr;      // per sh entry: id, slot, node's first slot, end slot
   const uint16_t *node_end = nullptr;
+  // per patch node at stride patch_threads (pack_patch_nodes): node id and
+  // slot with its first-slot bit, end of the node's entry run
+  const int2 *pnode = nullptr;
+  const uint16_t *pend = nullptr;
   const int32_t *slot_start = nullptr;
   const uint8_t *fixmask = nullptr;   // per internal node: bit c = component c fixed
   const int32_t *elems_int = nullptr; // ne x npe internal node ids

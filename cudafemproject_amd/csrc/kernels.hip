@@ -54,9 +54,8 @@ template <typename T>
 struct PatchArgs {
   const PatchInfo *patches;
   const uint32_t *erec;        // 3 words per element (pack_records)
-  const int32_t *sh_gid;
-  const int32_t *sh_slot;
-  const uint16_t *node_end;
+  const int2 *pnode;       // per patch node at stride PT: id, slot | first-slot bit
+  const uint16_t *pend;    // per patch node at stride PT: end of its entry run
   const uint8_t *fixmask;
   const int32_t *slot_start;
   const T *coords;
@@ -196,6 +195,14 @@ void k_ebe_patch(PatchArgs<T> a) {
     if (stop) return;
     beta = T(bt);
   }
+  // this thread's entries of the patch node tables, at a fixed stride: they
+  // do not wait for the PatchInfo load, so both are in flight together and
+  // the node data below is one dependent load away
+  const int2 *pnb = a.pnode + size_t(pid) * PT;
+  const uint16_t *peb = a.pend + size_t(pid) * PT;
+  const int2 pnd = pnb[threadIdx.x];
+  const int e1 = int(peb[threadIdx.x]);
+  const int e0 = threadIdx.x ? int(peb[threadIdx.x - 1]) : 0;
   const PatchInfo P = a.patches[pid];
   const int nloc = P.n_own + P.n_sh;
   double acc_en = 0.0;
@@ -225,54 +232,55 @@ void k_ebe_patch(PatchArgs<T> a) {
   const int k = threadIdx.x;
   const bool has_node = k < nloc;
   const bool own = k < P.n_own;
-  int64_t gid = 0;
-  int64_t slot = 0;
+  unsigned gid = 0;
+  // element offsets of the node and its slot (32-bit: few live registers)
+  unsigned go = 0, so = 0;
   int b0 = 0, b1 = 0;
   unsigned fm = 0;
   T xin[DIM];
 #pragma unroll
   for (int c = 0; c < DIM; ++c) xin[c] = T(0);
-  // MODE 4: the node's acceleration and velocity entering this step, and
-  // whether this patch holds the node's first slot
+  // whether this patch holds the node's first slot (owned nodes: their only
+  // patch); MODE 4: the node's acceleration and velocity entering this step
+  const bool first = own || pnd.y < 0;
   T av4[DIM], vv4[DIM];
-  bool first = own;
   int step4 = 0;
   if constexpr (MODE == 4) step4 = *a.step_base + a.step_off;
   if (has_node) {
-    gid = own ? int64_t(P.own_start + k) : int64_t(a.sh_gid[P.sh_start + k - P.n_own]);
-    if (!own) slot = a.sh_slot[P.sh_start + k - P.n_own];
-    b1 = int(a.node_end[P.pnode_start + k]);
-    b0 = k ? int(a.node_end[P.pnode_start + k - 1]) : 0;
+    gid = unsigned(pnd.x);
+    go = gid * unsigned(DIM);
+    if (!own) so = (unsigned(pnd.y) & 0x7fffffffu) * unsigned(DIM);
+    b1 = e1;
+    b0 = e0;
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) s_c[k * DIM + c] = a.coords[gid * DIM + c];
+    for (int c = 0; c < DIM; ++c) s_c[k * DIM + c] = a.coords[go + c];
     if constexpr (MODE != 2) fm = a.fixmask[gid] & FMASK;
     if constexpr (MODE == 0) {
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) xin[c] = a.x[gid * DIM + c];
+      for (int c = 0; c < DIM; ++c) xin[c] = a.x[go + c];
     } else if constexpr (MODE == 1) {
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) xin[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
+      for (int c = 0; c < DIM; ++c) xin[c] = a.x[go + c] + beta * a.p[go + c];
       if (own) {
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) a.p[gid * DIM + c] = xin[c];
+        for (int c = 0; c < DIM; ++c) a.p[go + c] = xin[c];
       }
     } else if constexpr (MODE == 3) {
       // the order of k_dyn_predict: (d + dt v) + c1 a
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        const T dv = a.x[gid * DIM + c] + a.dt * a.vel[gid * DIM + c];
-        xin[c] = dv + a.c1 * a.acc[gid * DIM + c];
+        const T dv = a.x[go + c] + a.dt * a.vel[go + c];
+        xin[c] = dv + a.c1 * a.acc[go + c];
       }
     } else if constexpr (MODE == 4) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        av4[c] = a.acc[gid * DIM + c];
-        vv4[c] = a.vel[gid * DIM + c];
+        av4[c] = a.acc[go + c];
+        vv4[c] = a.vel[go + c];
       }
       if (!own) {
-        const int64_t s = gid - a.n_owned;
+        const int64_t s = int64_t(gid) - a.n_owned;
         const int s0 = a.slot_start[s], s1 = a.slot_start[s + 1];
-        first = slot == s0;
         if (step4 > 1) {
           // the previous step of this interface node, from its slots
           T kd[DIM];
@@ -282,13 +290,13 @@ void k_ebe_patch(PatchArgs<T> a) {
           const T cf = T(a.coef_file[step4 - 1]), cu = T(a.coef_unit[step4 - 1]);
 #pragma unroll
           for (int c = 0; c < DIM; ++c)
-            gn22_correct<T>(cf, cu, a.f_file[gid * DIM + c], a.f_unit[gid * DIM + c], kd[c],
-                            a.invM[gid * DIM + c], a.c2, a.c3, a.alpha, a.scale, av4[c], vv4[c]);
+            gn22_correct<T>(cf, cu, a.f_file[go + c], a.f_unit[go + c], kd[c],
+                            a.invM[go + c], a.c2, a.c3, a.alpha, a.scale, av4[c], vv4[c]);
         }
       }
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        const T dv = a.x[gid * DIM + c] + a.dt * vv4[c];
+        const T dv = a.x[go + c] + a.dt * vv4[c];
         xin[c] = dv + a.c1 * av4[c];
       }
     }
@@ -297,16 +305,12 @@ void k_ebe_patch(PatchArgs<T> a) {
       for (int c = 0; c < DIM; ++c) s_x[k * DIM + c] = ((fm >> c) & 1u) ? T(0) : xin[c];
     }
   }
-  // interface nodes: does this patch hold the node's first slot?  (The
-  // first patch writes the constrained rows and, in the PCG step, adds the
-  // node's mass term of p'(cK K + cM M) p.)
-  if constexpr (MODE == 1) {
-    if (has_node && !own && (a.sh.m || fm)) first = slot == a.slot_start[gid - a.n_owned];
-  } else if constexpr (MODE == 0) {
-    if (has_node && !own && fm) first = slot == a.slot_start[gid - a.n_owned];
-  } else if constexpr (MODE == 3) {
-    if (has_node && !own) first = slot == a.slot_start[gid - a.n_owned];
-  }
+  // the fixed components and the first-slot flag travel through the
+  // assembly in one register (an interface node's first patch writes its
+  // constrained rows and its predicted state and, in the PCG step, adds its
+  // mass term of p'(cK K + cM M) p)
+  unsigned fmf = fm | (first ? 0x10u : 0u);
+  asm volatile("" : "+v"(fmf));  // opaque: not re-derived from its sources later
   __syncthreads();
 
   // Phases B and C: element forces, node-sorted LDS assembly, node sums.
@@ -316,6 +320,9 @@ void k_ebe_patch(PatchArgs<T> a) {
   // element by element: sum_e p_e'K_e p_e regrouped, with fewer operations.
   assemble<T, DIM, PT, MODE == 2, false>(s_c, s_x, s_res, rec, esc, P.n_elem, a.lam, a.mu, b0,
                                          b1, acc, acc_en, double(cK));
+  {
+  const unsigned fm = fmf & FMASK;
+  const bool first = (fmf & 0x10u) != 0u;
 
   if constexpr (MODE == 0 || MODE == 1) {
     if (has_node) {
@@ -326,11 +333,11 @@ void k_ebe_patch(PatchArgs<T> a) {
         for (int c = 0; c < DIM; ++c)
           if ((fm >> c) & 1u) {
             if constexpr (MODE == 0)
-              xin[c] = a.x[gid * DIM + c];
+              xin[c] = a.x[go + c];
             else if (own)
-              xin[c] = a.p[gid * DIM + c];
+              xin[c] = a.p[go + c];
             else
-              xin[c] = a.x[gid * DIM + c] + beta * a.p[gid * DIM + c];
+              xin[c] = a.x[go + c] + beta * a.p[go + c];
           }
       }
     }
@@ -339,7 +346,7 @@ void k_ebe_patch(PatchArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
           const double pv = double(xin[c]);
-          acc_en += double(cM) * double(a.sh.m[gid * DIM + c]) * pv * pv;
+          acc_en += double(cM) * double(a.sh.m[go + c]) * pv * pv;
         }
       }
     }
@@ -353,7 +360,7 @@ void k_ebe_patch(PatchArgs<T> a) {
       if (fm) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-          if ((fm >> c) & 1u) acc[c] = first ? a.diagK[gid * DIM + c] * xin[c] : T(0);
+          if ((fm >> c) & 1u) acc[c] = first ? a.diagK[go + c] * xin[c] : T(0);
       }
     }
     if constexpr (MODE == 4) {
@@ -361,18 +368,18 @@ void k_ebe_patch(PatchArgs<T> a) {
         const T cf = T(a.coef_file[step4]), cu = T(a.coef_unit[step4]);
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-          gn22_correct<T>(cf, cu, a.f_file[gid * DIM + c], a.f_unit[gid * DIM + c], acc[c],
-                          a.invM[gid * DIM + c], a.c2, a.c3, a.alpha, a.scale, av4[c], vv4[c]);
+          gn22_correct<T>(cf, cu, a.f_file[go + c], a.f_unit[go + c], acc[c],
+                          a.invM[go + c], a.c2, a.c3, a.alpha, a.scale, av4[c], vv4[c]);
       } else {
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) a.partial[slot * DIM + c] = acc[c];
+        for (int c = 0; c < DIM; ++c) a.partial[so + c] = acc[c];
       }
       if (first) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
-          a.aout[gid * DIM + c] = av4[c];
-          a.vout[gid * DIM + c] = vv4[c];
-          a.xout[gid * DIM + c] = xin[c];
+          a.aout[go + c] = av4[c];
+          a.vout[go + c] = vv4[c];
+          a.xout[go + c] = xin[c];
         }
       }
     } else if (own) {
@@ -381,24 +388,25 @@ void k_ebe_patch(PatchArgs<T> a) {
         T v = acc[c];
         if constexpr (MODE == 1) {
           v = cK * v;
-          if (a.sh.m) v += cM * a.sh.m[gid * DIM + c] * xin[c];
+          if (a.sh.m) v += cM * a.sh.m[go + c] * xin[c];
           acc_en += double(xin[c]) * double(v);
         }
-        a.y[gid * DIM + c] = v;
+        a.y[go + c] = v;
       }
     } else {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
         const T v = MODE == 1 ? cK * acc[c] : acc[c];
         if constexpr (MODE == 1) acc_en += double(xin[c]) * double(v);
-        a.partial[slot * DIM + c] = v;
+        a.partial[so + c] = v;
       }
     }
     if constexpr (MODE == 3) {
       if (first)
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) a.xout[gid * DIM + c] = xin[c];
+        for (int c = 0; c < DIM; ++c) a.xout[go + c] = xin[c];
     }
+  }
   }
 
   if constexpr (MODE == 1) {
@@ -1035,9 +1043,8 @@ PatchArgs<T> patch_args(const DevMesh &dm) {
   PatchArgs<T> a{};
   a.patches = dm.patches;
   a.erec = dm.erec;
-  a.sh_gid = dm.sh_gid;
-  a.sh_slot = dm.sh_slot;
-  a.node_end = dm.node_end;
+  a.pnode = dm.pnode;
+  a.pend = dm.pend;
   a.fixmask = dm.fixmask;
   a.slot_start = dm.slot_start;
   if constexpr (sizeof(T) == 8) {

@@ -39,12 +39,23 @@ int main(int argc, char** argv) {
   for (long e=0;e<ne;++e) for(int a=0;a<npe;++a) for(int c=0;c<dim;++c) ydir[elems[e*npe+a]*dim+c]+=fe[(e*npe+a)*dim+c];
   std::vector<double> y(nn*dim, 0.0), part(P.n_slots*dim + 1, NAN);
   std::vector<int> slot_written(P.n_slots, 0);
+  // the per-patch node tables the matvec kernel reads at stride pn_max (its
+  // workgroup size): every thread k < pn_max of patch p reads entry p*pn_max+k
+  std::vector<int32_t> pnode; std::vector<uint16_t> pend;
+  pack_patch_nodes(P, pn_max, pnode, pend);
+  CHK(pnode.size() == P.patches.size() * size_t(pn_max) * 2 && pend.size() == P.patches.size() * size_t(pn_max));
   CHK(P.node_end.size() == (size_t)std::accumulate(P.patches.begin(), P.patches.end(), 0L, [](long s, const PatchInfo& q){return s + q.n_own + q.n_sh;}));
   for (size_t p=0;p<P.patches.size();++p) {
     const PatchInfo& Q = P.patches[p];
     int nloc = Q.n_own + Q.n_sh; CHK(nloc <= P.pn_used && Q.n_elem <= P.emax_used);
     std::vector<long> gidl(nloc);
-    for (int k=0;k<nloc;++k){ long gid = k<Q.n_own ? Q.own_start+k : P.sh_gid.at(Q.sh_start+k-Q.n_own); CHK(gid>=0 && gid<nn); gidl[k]=gid; CHK(Q.pnode_start + k < (long)P.node_end.size()); }
+    CHK(nloc <= pn_max);
+    for (int k=0;k<nloc;++k){ long gid = k<Q.n_own ? Q.own_start+k : P.sh_gid.at(Q.sh_start+k-Q.n_own); CHK(gid>=0 && gid<nn); gidl[k]=gid; CHK(Q.pnode_start + k < (long)P.node_end.size());
+      size_t ti = p*size_t(pn_max)+k; CHK(pnode[2*ti] == gid); CHK(pend[ti] == P.node_end[Q.pnode_start+k]);
+      if (k<Q.n_own) CHK(pnode[2*ti+1] == 0);
+      else { uint32_t w = uint32_t(pnode[2*ti+1]); long slot = long(w & 0x7fffffffu); CHK(slot == P.sh_slot[Q.sh_start+k-Q.n_own]);
+             CHK(((w >> 31) != 0) == (slot == P.slot_start[gid-P.n_owned])); } }
+    for (int k=nloc;k<pn_max;++k){ size_t ti = p*size_t(pn_max)+k; CHK(pnode[2*ti] == 0 && pnode[2*ti+1] == 0 && pend[ti] == 0); }
     // the kernel zeroes its entry list first; padding entries stay zero
     CHK(P.ent_used <= 16383); std::vector<double> res(P.ent_used*dim, 0.0); std::vector<int> written(P.ent_used, 0);
     for (int i=0;i<Q.n_elem;++i){ long ie = Q.elem_start+i; CHK(ie < ne); long ue = P.elem_perm[ie];
@@ -56,12 +67,12 @@ int main(int argc, char** argv) {
         int pos = Pq[a]; CHK(pos == P.epos[ie*4+a]); CHK(pos >= 0 && pos < P.ent_used); CHK(!written[pos]); written[pos]=1;
         { int bb0 = l? P.node_end[Q.pnode_start+l-1]:0, bb1 = P.node_end[Q.pnode_start+l]; CHK(pos >= bb0 && pos < bb1); }
         for(int c=0;c<dim;++c) res[pos*dim+c] = fe[(ue*npe+a)*dim+c]; } }
-    for (int k=0;k<nloc;++k){ int b0 = k? P.node_end[Q.pnode_start+k-1]:0, b1=P.node_end[Q.pnode_start+k]; CHK(b0<=b1 && b1 <= P.ent_used); CHK(b0 % 4 == 0 && b1 % 4 == 0);
+    for (int k=0;k<nloc;++k){ size_t ti = p*size_t(pn_max)+k; int b0 = k? pend[ti-1]:0, b1=pend[ti]; CHK(b0<=b1 && b1 <= P.ent_used); CHK(b0 % 4 == 0 && b1 % 4 == 0);
       { int nw = 0; for (int j=b0;j<b1;++j) nw += written[j]; CHK(nw > 0 && b1 - b0 - nw < 4); }
       double acc[3]={0,0,0};
       for (int j=b0;j<b1;++j){ for(int c=0;c<dim;++c){ double v=res[j*dim+c]; acc[c]+=v; } }
-      if (k<Q.n_own){ long gid=Q.own_start+k; CHK(gid < P.n_owned); for(int c=0;c<dim;++c) y[gid*dim+c]=acc[c]; }
-      else { long slot=P.sh_slot.at(Q.sh_start+k-Q.n_own); CHK(slot>=0 && slot<P.n_slots); CHK(!slot_written[slot]); slot_written[slot]=1; for(int c=0;c<dim;++c) part[slot*dim+c]=acc[c]; } }
+      if (k<Q.n_own){ long gid=pnode[2*ti]; CHK(gid < P.n_owned); for(int c=0;c<dim;++c) y[gid*dim+c]=acc[c]; }
+      else { long slot=long(uint32_t(pnode[2*ti+1]) & 0x7fffffffu); CHK(slot>=0 && slot<P.n_slots); CHK(!slot_written[slot]); slot_written[slot]=1; for(int c=0;c<dim;++c) part[slot*dim+c]=acc[c]; } }
   }
   for (long s=0;s<P.n_iface;++s){ double acc[3]={0,0,0}; for(int j=P.slot_start[s]; j<P.slot_start[s+1]; ++j){ CHK(slot_written[j]); for(int c=0;c<dim;++c) acc[c]+=part[j*dim+c]; } for(int c=0;c<dim;++c) y[(P.n_owned+s)*dim+c]=acc[c]; }
   double err=0; for (long i=0;i<nn;++i) for(int c=0;c<dim;++c) err=std::max(err, std::fabs(y[i*dim+c]-ydir[P.node_perm[i]*dim+c]));
