@@ -867,6 +867,10 @@ int upload_plan(femgpu_ctx *ctx) {
   uint16_t *ne_;
   std::vector<uint32_t> recs;
   femgpu::pack_records(P, recs);
+  // records at stride emax per patch for the patch kernels: the same array
+  // (padded by one patch) when every patch starts at p * emax
+  std::vector<uint32_t> recs_s;
+  const bool regular = femgpu::stride_records(P, recs, recs_s);
   if ((st = upload(ctx, "patches", P.patches, &pp)) || (st = upload(ctx, "erec", recs, &rec)) ||
       (st = upload(ctx, "sh_gid", P.sh_gid, &shg)) || (st = upload(ctx, "sh_slot", P.sh_slot, &shs)) ||
       (st = upload(ctx, "node_end", P.node_end, &ne_)) ||
@@ -880,6 +884,12 @@ int upload_plan(femgpu_ctx *ctx) {
   }
   dm.patches = pp;
   dm.erec = rec;
+  dm.erec_s = rec;
+  if (!regular) {
+    uint32_t *rs;
+    if ((st = upload(ctx, "erec_s", recs_s, &rs))) return st;
+    dm.erec_s = rs;
+  }
   dm.sh_gid = shg;
   dm.sh_slot = shs;
   {
@@ -898,13 +908,16 @@ int upload_plan(femgpu_ctx *ctx) {
   }
   dm.node_end = ne_;
   {
-    std::vector<int32_t> pnode;
+    std::vector<int32_t> pgid, pslot;
     std::vector<uint16_t> pend;
-    femgpu::pack_patch_nodes(P, threads, pnode, pend);
-    int32_t *pn_;
+    femgpu::pack_patch_nodes(P, threads, pgid, pslot, pend);
+    int32_t *pg_, *ps_;
     uint16_t *pe_;
-    if ((st = upload(ctx, "pnode", pnode, &pn_)) || (st = upload(ctx, "pend", pend, &pe_))) return st;
-    dm.pnode = reinterpret_cast<const int2 *>(pn_);
+    if ((st = upload(ctx, "pgid", pgid, &pg_)) || (st = upload(ctx, "pslot", pslot, &ps_)) ||
+        (st = upload(ctx, "pend", pend, &pe_)))
+      return st;
+    dm.pgid = pg_;
+    dm.pslot = ps_;
     dm.pend = pe_;
   }
   dm.slot_start = ss;

@@ -82,14 +82,21 @@ std::string build_plan(int dim, int64_t nn, const double *nodes, int64_t ne,
 //   w2 = p1 >> 10 | p2 << 4 | p3 << 18
 void pack_records(const MeshPlan &plan, std::vector<uint32_t> &rec);
 
+// The records (pack_records) at a fixed stride per patch, patch p's at
+// p * emax_used with zero padding, for kernels that address them without
+// the PatchInfo.  When every patch starts at p * emax_used, rec itself has
+// that layout: it gets one patch of zero padding at the end and the function
+// returns true; otherwise rec_s receives the strided copy (false).
+bool stride_records(const MeshPlan &plan, std::vector<uint32_t> &rec,
+                    std::vector<uint32_t> &rec_s);
+
 // Per-patch node tables at a fixed stride (entry p * stride + k for node k
 // of patch p, stride >= the patch's node count), so that a workgroup reads
-// them without first reading its PatchInfo: pnode holds 2 x int32 per
-// entry, the internal node id and, for an interface node, its slot with
-// bit 31 set when the slot is the node's first (0 for owned nodes); pend the
-// end of the node's run in the patch's entry list (node_end).  Entries
-// beyond a patch's nodes are zero.
-void pack_patch_nodes(const MeshPlan &plan, int stride, std::vector<int32_t> &pnode,
-                      std::vector<uint16_t> &pend);
+// them without first reading its PatchInfo: pgid the internal node id,
+// pslot for an interface node its slot with bit 31 set when the slot is the
+// node's first (0 for owned nodes), pend the end of the node's run in the
+// patch's entry list (node_end).  Entries beyond a patch's nodes are zero.
+void pack_patch_nodes(const MeshPlan &plan, int stride, std::vector<int32_t> &pgid,
+                      std::vector<int32_t> &pslot, std::vector<uint16_t> &pend);
 
 }  // namespace femgpu

@@ -20,13 +20,17 @@ struct DevMesh {
   int64_t nn = 0, ne = 0, n_owned = 0, n_iface = 0, n_active = 0;
   const PatchInfo *patches = nullptr;
   const uint32_t *erec = nullptr;      // 3 words per element (pack_records)
+  // the same records at stride emax per patch (patch p's at p * emax, zero
+  // padding): erec itself, padded at the end, when every patch starts there
+  const uint32_t *erec_s = nullptr;
   const int32_t *sh_gid = nullptr;
   const int32_t *sh_slot = nullptr;
   const int4 *sh_info = nullptr;      // per sh entry: id, slot, node's first slot, end slot
   const uint16_t *node_end = nullptr;
-  // per patch node at stride patch_threads (pack_patch_nodes): node id and
+  // per patch node at stride patch_threads (pack_patch_nodes): node id,
   // slot with its first-slot bit, end of the node's entry run
-  const int2 *pnode = nullptr;
+  const int32_t *pgid = nullptr;
+  const int32_t *pslot = nullptr;
   const uint16_t *pend = nullptr;
   const int32_t *slot_start = nullptr;
   const uint8_t *fixmask = nullptr;   // per internal node: bit c = component c fixed

@@ -53,8 +53,10 @@ using namespace patch;
 template <typename T>
 struct PatchArgs {
   const PatchInfo *patches;
-  const uint32_t *erec;        // 3 words per element (pack_records)
-  const int2 *pnode;       // per patch node at stride PT: id, slot | first-slot bit
+  const uint32_t *erec_s;      // 3 words per element, patch p's at p * emax (pack_records)
+  int emax;                    // record stride per patch
+  const int32_t *pgid;     // per patch node at stride PT: node id
+  const int32_t *pslot;    // per patch node at stride PT: slot | first-slot bit
   const uint16_t *pend;    // per patch node at stride PT: end of its entry run
   const uint8_t *fixmask;
   const int32_t *slot_start;
@@ -195,14 +197,17 @@ void k_ebe_patch(PatchArgs<T> a) {
     if (stop) return;
     beta = T(bt);
   }
-  // this thread's entries of the patch node tables, at a fixed stride: they
-  // do not wait for the PatchInfo load, so both are in flight together and
-  // the node data below is one dependent load away
-  const int2 *pnb = a.pnode + size_t(pid) * PT;
-  const uint16_t *peb = a.pend + size_t(pid) * PT;
-  const int2 pnd = pnb[threadIdx.x];
-  const int e1 = int(peb[threadIdx.x]);
-  const int e0 = threadIdx.x ? int(peb[threadIdx.x - 1]) : 0;
+  // this thread's node id and run end from the patch node tables, at a
+  // fixed stride: they do not wait for the PatchInfo load, so both are in
+  // flight together and the node data below is one dependent load away
+  // (threads past the plan's largest patch have no node in any patch)
+  const size_t pbase = size_t(pid) * PT;
+  int gid_t = 0, e1 = 0, e0 = 0;
+  if (threadIdx.x < unsigned(a.pn)) {
+    gid_t = a.pgid[pbase + threadIdx.x];
+    e1 = int(a.pend[pbase + threadIdx.x]);
+    e0 = threadIdx.x ? int(a.pend[pbase + threadIdx.x - 1]) : 0;
+  }
   const PatchInfo P = a.patches[pid];
   const int nloc = P.n_own + P.n_sh;
   double acc_en = 0.0;
@@ -210,17 +215,23 @@ void k_ebe_patch(PatchArgs<T> a) {
   const T cM = MODE == 1 ? a.sh.cM : T(0);
 
   // Element records of this thread, loaded first so their latency overlaps
-  // the node staging.
+  // the node staging.  They lie at a fixed stride per patch (zero padding
+  // past the patch's elements), so they too are in flight before the
+  // PatchInfo arrives.
   uint3 rec[EPT];
   T esc[EPT];
   // per-patch bases (uniform) and small unsigned thread offsets: scalar
   // base + one offset register per load
-  const uint3 *prec = reinterpret_cast<const uint3 *>(a.erec) + P.elem_start;
+  const uint3 *prec = reinterpret_cast<const uint3 *>(a.erec_s) + size_t(pid) * unsigned(a.emax);
   const T *pesc = a.escale ? a.escale + P.elem_start : nullptr;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int i = threadIdx.x + e * PT;
-    rec[e] = i < P.n_elem ? prec[unsigned(i)] : make_uint3(0u, 0u, 0u);
+    rec[e] = i < a.emax ? prec[unsigned(i)] : make_uint3(0u, 0u, 0u);
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = threadIdx.x + e * PT;
     esc[e] = pesc && i < P.n_elem ? pesc[unsigned(i)] : T(1);
   }
 
@@ -240,16 +251,18 @@ void k_ebe_patch(PatchArgs<T> a) {
   T xin[DIM];
 #pragma unroll
   for (int c = 0; c < DIM; ++c) xin[c] = T(0);
-  // whether this patch holds the node's first slot (owned nodes: their only
-  // patch); MODE 4: the node's acceleration and velocity entering this step
-  const bool first = own || pnd.y < 0;
+  // interface nodes: their slot, and whether this patch holds the node's
+  // first slot (owned nodes: their only patch); MODE 4: the node's
+  // acceleration and velocity entering this step
+  const int slot_t = has_node && !own ? a.pslot[pbase + k] : 0;
+  const bool first = own || slot_t < 0;
   T av4[DIM], vv4[DIM];
   int step4 = 0;
   if constexpr (MODE == 4) step4 = *a.step_base + a.step_off;
   if (has_node) {
-    gid = unsigned(pnd.x);
+    gid = unsigned(gid_t);
     go = gid * unsigned(DIM);
-    if (!own) so = (unsigned(pnd.y) & 0x7fffffffu) * unsigned(DIM);
+    if (!own) so = (unsigned(slot_t) & 0x7fffffffu) * unsigned(DIM);
     b1 = e1;
     b0 = e0;
 #pragma unroll
@@ -1042,8 +1055,10 @@ template <typename T>
 PatchArgs<T> patch_args(const DevMesh &dm) {
   PatchArgs<T> a{};
   a.patches = dm.patches;
-  a.erec = dm.erec;
-  a.pnode = dm.pnode;
+  a.erec_s = dm.erec_s;
+  a.emax = dm.emax;
+  a.pgid = dm.pgid;
+  a.pslot = dm.pslot;
   a.pend = dm.pend;
   a.fixmask = dm.fixmask;
   a.slot_start = dm.slot_start;

@@ -377,23 +377,43 @@ void pack_records(const MeshPlan &plan, std::vector<uint32_t> &rec) {
   }
 }
 
-void pack_patch_nodes(const MeshPlan &plan, int stride, std::vector<int32_t> &pnode,
-                      std::vector<uint16_t> &pend) {
+bool stride_records(const MeshPlan &plan, std::vector<uint32_t> &rec,
+                    std::vector<uint32_t> &rec_s) {
+  const size_t emax = size_t(plan.emax_used);
+  bool regular = true;
+  for (size_t p = 0; p < plan.patches.size(); ++p)
+    regular = regular && size_t(plan.patches[p].elem_start) == p * emax;
+  rec_s.clear();
+  if (regular) {
+    rec.resize(rec.size() + emax * 3, 0u);
+    return true;
+  }
+  rec_s.assign(plan.patches.size() * emax * 3, 0u);
+  for (size_t p = 0; p < plan.patches.size(); ++p)
+    std::copy(rec.begin() + size_t(plan.patches[p].elem_start) * 3,
+              rec.begin() + (size_t(plan.patches[p].elem_start) + plan.patches[p].n_elem) * 3,
+              rec_s.begin() + p * emax * 3);
+  return false;
+}
+
+void pack_patch_nodes(const MeshPlan &plan, int stride, std::vector<int32_t> &pgid,
+                      std::vector<int32_t> &pslot, std::vector<uint16_t> &pend) {
   const size_t np = plan.patches.size();
-  pnode.assign(np * size_t(stride) * 2, 0);
+  pgid.assign(np * size_t(stride), 0);
+  pslot.assign(np * size_t(stride), 0);
   pend.assign(np * size_t(stride), 0);
   for (size_t p = 0; p < np; ++p) {
     const PatchInfo &Q = plan.patches[p];
     for (int k = 0; k < Q.n_own + Q.n_sh; ++k) {
       const size_t i = p * size_t(stride) + size_t(k);
       if (k < Q.n_own) {
-        pnode[2 * i] = Q.own_start + k;
+        pgid[i] = Q.own_start + k;
       } else {
         const int32_t gid = plan.sh_gid[size_t(Q.sh_start + k - Q.n_own)];
         const int32_t slot = plan.sh_slot[size_t(Q.sh_start + k - Q.n_own)];
         const bool first = slot == plan.slot_start[size_t(gid - plan.n_owned)];
-        pnode[2 * i] = gid;
-        pnode[2 * i + 1] = int32_t(uint32_t(slot) | (first ? 0x80000000u : 0u));
+        pgid[i] = gid;
+        pslot[i] = int32_t(uint32_t(slot) | (first ? 0x80000000u : 0u));
       }
       pend[i] = plan.node_end[size_t(Q.pnode_start + k)];
     }

@@ -29,6 +29,13 @@ int main(int argc, char** argv) {
   int npe=dim+1; long nn=nodes.size()/dim, ne=elems.size()/npe;
   MeshPlan P; std::string m = build_plan(dim, nn, nodes.data(), ne, elems.data(), emax, pn_max, true, P);
   std::vector<uint32_t> rec; pack_records(P, rec); CHK(rec.size() == size_t(ne) * 3);
+  // the patch kernel reads record p*emax_used + i for every i < emax_used of
+  // patch p: in bounds, equal to the patch's records, zero past them
+  { std::vector<uint32_t> r2 = rec, rs; bool reg = stride_records(P, r2, rs); const std::vector<uint32_t>& A = reg ? r2 : rs;
+    size_t em = size_t(P.emax_used);
+    for (size_t p=0;p<P.patches.size();++p) for (size_t i=0;i<em;++i) { size_t w = (p*em + i)*3; CHK(w + 2 < A.size());
+      for (int j=0;j<3;++j) { if (long(i) < P.patches[p].n_elem) CHK(A[w+j] == rec[(size_t(P.patches[p].elem_start)+i)*3+j]); else if (!reg || p + 1 == P.patches.size()) CHK(A[w+j] == 0u); } }
+    printf("records %s\n", reg ? "regular" : "strided copy"); }
   if(!m.empty()){printf("plan error: %s\n", m.c_str()); return 1;}
   printf("nn=%ld ne=%ld np=%zu owned=%ld iface=%ld slots=%ld pn_used=%d emax_used=%d\n", nn, ne, P.patches.size(), (long)P.n_owned, (long)P.n_iface, (long)P.n_slots, P.pn_used, P.emax_used);
   // random element "forces": f[e][a][c] ; direct assembly vs plan assembly
@@ -40,10 +47,11 @@ int main(int argc, char** argv) {
   std::vector<double> y(nn*dim, 0.0), part(P.n_slots*dim + 1, NAN);
   std::vector<int> slot_written(P.n_slots, 0);
   // the per-patch node tables the matvec kernel reads at stride pn_max (its
-  // workgroup size): every thread k < pn_max of patch p reads entry p*pn_max+k
-  std::vector<int32_t> pnode; std::vector<uint16_t> pend;
-  pack_patch_nodes(P, pn_max, pnode, pend);
-  CHK(pnode.size() == P.patches.size() * size_t(pn_max) * 2 && pend.size() == P.patches.size() * size_t(pn_max));
+  // workgroup size): thread k of patch p reads entry p*pn_max+k (pgid, pend:
+  // every k below the largest patch's node count; pslot: interface nodes)
+  std::vector<int32_t> pgid, pslot; std::vector<uint16_t> pend;
+  pack_patch_nodes(P, pn_max, pgid, pslot, pend);
+  CHK(pgid.size() == P.patches.size() * size_t(pn_max) && pslot.size() == pgid.size() && pend.size() == pgid.size());
   CHK(P.node_end.size() == (size_t)std::accumulate(P.patches.begin(), P.patches.end(), 0L, [](long s, const PatchInfo& q){return s + q.n_own + q.n_sh;}));
   for (size_t p=0;p<P.patches.size();++p) {
     const PatchInfo& Q = P.patches[p];
@@ -51,11 +59,11 @@ int main(int argc, char** argv) {
     std::vector<long> gidl(nloc);
     CHK(nloc <= pn_max);
     for (int k=0;k<nloc;++k){ long gid = k<Q.n_own ? Q.own_start+k : P.sh_gid.at(Q.sh_start+k-Q.n_own); CHK(gid>=0 && gid<nn); gidl[k]=gid; CHK(Q.pnode_start + k < (long)P.node_end.size());
-      size_t ti = p*size_t(pn_max)+k; CHK(pnode[2*ti] == gid); CHK(pend[ti] == P.node_end[Q.pnode_start+k]);
-      if (k<Q.n_own) CHK(pnode[2*ti+1] == 0);
-      else { uint32_t w = uint32_t(pnode[2*ti+1]); long slot = long(w & 0x7fffffffu); CHK(slot == P.sh_slot[Q.sh_start+k-Q.n_own]);
+      size_t ti = p*size_t(pn_max)+k; CHK(pgid[ti] == gid); CHK(pend[ti] == P.node_end[Q.pnode_start+k]);
+      if (k<Q.n_own) CHK(pslot[ti] == 0);
+      else { uint32_t w = uint32_t(pslot[ti]); long slot = long(w & 0x7fffffffu); CHK(slot == P.sh_slot[Q.sh_start+k-Q.n_own]);
              CHK(((w >> 31) != 0) == (slot == P.slot_start[gid-P.n_owned])); } }
-    for (int k=nloc;k<pn_max;++k){ size_t ti = p*size_t(pn_max)+k; CHK(pnode[2*ti] == 0 && pnode[2*ti+1] == 0 && pend[ti] == 0); }
+    for (int k=nloc;k<pn_max;++k){ size_t ti = p*size_t(pn_max)+k; CHK(pgid[ti] == 0 && pslot[ti] == 0 && pend[ti] == 0); }
     // the kernel zeroes its entry list first; padding entries stay zero
     CHK(P.ent_used <= 16383); std::vector<double> res(P.ent_used*dim, 0.0); std::vector<int> written(P.ent_used, 0);
     for (int i=0;i<Q.n_elem;++i){ long ie = Q.elem_start+i; CHK(ie < ne); long ue = P.elem_perm[ie];
@@ -71,8 +79,8 @@ int main(int argc, char** argv) {
       { int nw = 0; for (int j=b0;j<b1;++j) nw += written[j]; CHK(nw > 0 && b1 - b0 - nw < 4); }
       double acc[3]={0,0,0};
       for (int j=b0;j<b1;++j){ for(int c=0;c<dim;++c){ double v=res[j*dim+c]; acc[c]+=v; } }
-      if (k<Q.n_own){ long gid=pnode[2*ti]; CHK(gid < P.n_owned); for(int c=0;c<dim;++c) y[gid*dim+c]=acc[c]; }
-      else { long slot=long(uint32_t(pnode[2*ti+1]) & 0x7fffffffu); CHK(slot>=0 && slot<P.n_slots); CHK(!slot_written[slot]); slot_written[slot]=1; for(int c=0;c<dim;++c) part[slot*dim+c]=acc[c]; } }
+      if (k<Q.n_own){ long gid=pgid[ti]; CHK(gid < P.n_owned); for(int c=0;c<dim;++c) y[gid*dim+c]=acc[c]; }
+      else { long slot=long(uint32_t(pslot[ti]) & 0x7fffffffu); CHK(slot>=0 && slot<P.n_slots); CHK(!slot_written[slot]); slot_written[slot]=1; for(int c=0;c<dim;++c) part[slot*dim+c]=acc[c]; } }
   }
   for (long s=0;s<P.n_iface;++s){ double acc[3]={0,0,0}; for(int j=P.slot_start[s]; j<P.slot_start[s+1]; ++j){ CHK(slot_written[j]); for(int c=0;c<dim;++c) acc[c]+=part[j*dim+c]; } for(int c=0;c<dim;++c) y[(P.n_owned+s)*dim+c]=acc[c]; }
   double err=0; for (long i=0;i<nn;++i) for(int c=0;c<dim;++c) err=std::max(err, std::fabs(y[i*dim+c]-ydir[P.node_perm[i]*dim+c]));
