@@ -138,6 +138,18 @@ __device__ __forceinline__ void block_sum_n(double (&v)[N], double *sh) {
   waves_sum_n<BT, N>(v, sh);
 }
 
+// Product rounded on its own (never fused into a following add).
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+
+// Jacobi-PCG search direction p' = M^-1 r + beta p (invD = M^-1), formed
+// the same way wherever it is needed, so that every copy has the same bits:
+// z = invD r rounded, then one fused multiply-add.
+template <typename T>
+__device__ __forceinline__ T pcg_dir(T invd, T r, T beta, T p) {
+  return fma(beta, p, mul_rn(invd, r));
+}
+
 // ---------------------------------------------------------------------------
 // Element kernels (registers only).  X, U: node-major (npe x dim).
 // fix: bit (dim*a + c) set when that local DOF is constrained.

@@ -420,7 +420,7 @@ int poll(femgpu_ctx *ctx, const CgScalars *d_sc) {
 // Workspace of one PCG precision.
 template <typename T>
 struct PcgWs {
-  T *x, *r, *z, *p, *q, *partial;
+  T *x, *r, *p, *q, *partial;
   const T *invD;
 };
 
@@ -430,7 +430,7 @@ int pcg_ws(femgpu_ctx *ctx, const char *tag, PcgWs<T> &w) {
   std::string t(tag);
   int st;
   if ((st = dbuf(ctx, t + "x", nd, &w.x)) || (st = dbuf(ctx, t + "r", nd, &w.r)) ||
-      (st = dbuf(ctx, t + "z", nd, &w.z)) || (st = dbuf(ctx, t + "p", nd, &w.p)) ||
+      (st = dbuf(ctx, t + "p", nd, &w.p)) ||
       (st = dbuf(ctx, t + "q", nd, &w.q)) ||
       (st = dbuf(ctx, t + "partial", size_t(ctx->plan.n_slots) * ctx->dim + 1, &w.partial)))
     return st;
@@ -469,7 +469,7 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
     out->exec = nullptr;
     out->eager = [=]() {
       for (int k = 0; k < block; ++k)
-        femgpu::launch_pcg_iter<T>(c->dm, ws.x, ws.r, ws.z, ws.p, ws.q, ws.partial, ws.invD, sh,
+        femgpu::launch_pcg_iter<T>(c->dm, ws.x, ws.r, ws.p, ws.q, ws.partial, ws.invD, sh,
                                    sc, part_patch, part_node, k & 1, c->stream);
     };
     return FEMGPU_OK;
@@ -482,8 +482,8 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
   // (besides the mesh, whose change drops every graph)
   char keybuf[320];
   std::snprintf(keybuf, sizeof(keybuf),
-                "pcg/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block,
-                (const void *)w.x, (const void *)w.r, (const void *)w.z, (const void *)w.p,
+                "pcg/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block,
+                (const void *)w.x, (const void *)w.r, (const void *)w.p,
                 (const void *)w.q, (const void *)w.partial, (const void *)w.invD, (const void *)sc,
                 (const void *)sh.m, (const void *)part_patch, (const void *)part_node,
                 double(sh.cK), double(sh.cM));
@@ -497,7 +497,7 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
   hipGraphExec_t exec = nullptr;
   HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < block; ++k)
-    femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.z, w.p, w.q, w.partial, w.invD, sh, sc,
+    femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.p, w.q, w.partial, w.invD, sh, sc,
                                part_patch, part_node, k & 1, ctx->stream);
   HIPCHK(hipStreamEndCapture(ctx->stream, &g));
   HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
@@ -641,7 +641,7 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
   femgpu::launch_fill<T>(w.x, nd, T(0), ctx->stream);
   femgpu::launch_fill<T>(w.p, nd, T(0), ctx->stream);
   femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
-  femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.z, w.invD, sc, part_node, ctx->stream);
+  femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.invD, sc, part_node, ctx->stream);
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
   const int tb = tail_block_of(block);
@@ -1711,7 +1711,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     femgpu::launch_fill<float>(w.x, int64_t(nd), 0.f, ctx->stream);
     femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
     femgpu::launch_reset_scalars(sc, FEMGPU_TOL_REL_RES, o.tol, max_iter, ctx->stream);
-    femgpu::launch_pcg_replace(ctx->dm, f64, w.r, w.z, w.invD, part_node, ctx->stream);
+    femgpu::launch_pcg_replace(ctx->dm, f64, w.r, w.invD, part_node, ctx->stream);
     femgpu::launch_fin_init(sc, part_node, ctx->stream);
     HIPCHK(hipGetLastError());
     if ((st = poll(ctx, sc))) return st;
@@ -1776,7 +1776,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
       // had drifted far from the FP64 residual (more than twice it) or the
       // FP64 residual did not at least halve since the previous replacement:
       // then the directions belong to another residual.
-      femgpu::launch_pcg_replace(ctx->dm, r64, w.r, w.z, w.invD, part_node, ctx->stream);
+      femgpu::launch_pcg_replace(ctx->dm, r64, w.r, w.invD, part_node, ctx->stream);
       const bool restart = rel > 2.0 * rel_rec || rel > 0.5 * rel_prev || retry;
       if (restart) femgpu::launch_fill<float>(w.p, int64_t(nd), 0.f, ctx->stream);
       kept_p = !restart;

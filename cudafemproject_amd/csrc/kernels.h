@@ -101,16 +101,17 @@ template <typename T>
 void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T *invD,
                      hipStream_t s);
 
-// PCG: init (z = invD r, gamma0) and one fused iteration.
+// PCG: init (gamma0 = r'z with z = invD r) and one fused iteration (z is
+// formed from r where it is needed, never stored).
 template <typename T>
-void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScalars *sc,
+void launch_pcg_init(const DevMesh &dm, const T *r, const T *invD, CgScalars *sc,
                      double *part_node, hipStream_t s);
 // parity: j & 1 of the iteration (ping-pong of scalars and partial buffers).
 // part_patch: >= np doubles; part_node: 4 * NODE_PARTS doubles.
 template <typename T>
-void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial,
-                     const T *invD, OpShift<T> sh, CgScalars *sc, double *part_patch,
-                     double *part_node, int parity, hipStream_t s);
+void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, const T *invD,
+                     OpShift<T> sh, CgScalars *sc, double *part_patch, double *part_node,
+                     int parity, hipStream_t s);
 constexpr int NODE_PARTS = 512;  // blocks of the node-parallel PCG kernels
 static_assert(NODE_PARTS % 256 == 0, "the patch kernels read the partials in chunks of 256");
 
@@ -118,11 +119,11 @@ static_assert(NODE_PARTS % 256 == 0, "the patch kernels read the partials in chu
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
                        double *r, CgScalars *sc, double *part, hipStream_t s);
 // Mixed solve (reliable updates): x64 += x32, x32 = 0; residual replacement
-// r32 = r64, z32 = invD r32 with the partials the next iteration reads;
-// resume after a stop flag.
+// r32 = r64 with the partials the next iteration reads; resume after a stop
+// flag.
 void launch_flush_x(const DevMesh &dm, double *x64, float *x32, hipStream_t s);
-void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, float *z32,
-                        const float *invD, double *part_node, hipStream_t s);
+void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, const float *invD,
+                        double *part_node, hipStream_t s);
 // Clear the stop flag for the next replay (parity 0) after a residual
 // replacement: the scalars of the stopping launch move to slot 1.
 void launch_pcg_resume(CgScalars *sc, hipStream_t s);

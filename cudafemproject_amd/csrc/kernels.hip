@@ -66,7 +66,8 @@ struct PatchArgs {
   T lam, mu;
   int np, p8, ent, pn;
   int64_t n_owned;
-  const T *x;   // MODE 0: x; MODE 1: z
+  const T *x;   // MODE 0: x; MODE 1: r (z = invD r is formed here)
+  const T *invD;      // MODE 1: Jacobi preconditioner
   T *p;         // MODE 1: search direction (old in, new out for owned nodes)
   T *y;         // owned-node output
   T *partial;   // interface slots
@@ -273,7 +274,7 @@ void k_ebe_patch(PatchArgs<T> a) {
       for (int c = 0; c < DIM; ++c) xin[c] = a.x[go + c];
     } else if constexpr (MODE == 1) {
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) xin[c] = a.x[go + c] + beta * a.p[go + c];
+      for (int c = 0; c < DIM; ++c) xin[c] = pcg_dir(a.invD[go + c], a.x[go + c], beta, a.p[go + c]);
       if (own) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) a.p[go + c] = xin[c];
@@ -350,7 +351,7 @@ void k_ebe_patch(PatchArgs<T> a) {
             else if (own)
               xin[c] = a.p[go + c];
             else
-              xin[c] = a.x[go + c] + beta * a.p[go + c];
+              xin[c] = pcg_dir(a.invD[go + c], a.x[go + c], beta, a.p[go + c]);
           }
       }
     }
@@ -452,7 +453,7 @@ struct UpdArgs {
   const int32_t *slot_start;
   const T *partial;
   const T *q;
-  T *p, *x, *r, *z;
+  T *p, *x, *r;
   const T *invD;
   OpShift<T> sh;
   CgScalars *sc;
@@ -466,7 +467,7 @@ constexpr int UPT = 512;  // threads per block of the node-parallel PCG kernels
 // One node of the PCG update, loaded before alpha is known.
 template <typename T, int DIM>
 struct UpdNode {
-  T q[DIM], p[DIM], z[DIM], x[DIM], r[DIM], d[DIM];
+  T q[DIM], p[DIM], x[DIM], r[DIM], d[DIM];
 };
 
 template <typename T, int DIM>
@@ -474,17 +475,11 @@ __device__ __forceinline__ void upd_load(const UpdArgs<T> &a, int64_t i, UpdNode
   const int64_t o = i * DIM;
   if (i < a.n_owned) {
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      n.q[c] = a.q[o + c];
-      n.z[c] = T(0);
-    }
+    for (int c = 0; c < DIM; ++c) n.q[c] = a.q[o + c];
   } else {
     const int64_t s = i - a.n_owned;
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      n.q[c] = T(0);
-      n.z[c] = a.z[o + c];
-    }
+    for (int c = 0; c < DIM; ++c) n.q[c] = T(0);
     add_slots<T, DIM>(a.partial, a.slot_start[s], a.slot_start[s + 1], n.q);
   }
 #pragma unroll
@@ -503,7 +498,7 @@ __device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNo
   if (i >= a.n_owned) {
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-      n.p[c] = n.z[c] + beta * n.p[c];
+      n.p[c] = pcg_dir(n.d[c], n.r[c], beta, n.p[c]);
       a.p[o + c] = n.p[c];
       if (a.sh.m) n.q[c] += a.sh.cM * a.sh.m[o + c] * n.p[c];
     }
@@ -513,8 +508,7 @@ __device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNo
     a.x[o + c] = n.x[c] + alpha * n.p[c];
     const T rv = n.r[c] - alpha * n.q[c];
     a.r[o + c] = rv;
-    const T zv = n.d[c] * rv;
-    a.z[o + c] = zv;
+    const T zv = mul_rn(n.d[c], rv);
     rz += double(rv) * double(zv);
     rr += double(rv) * double(rv);
   }
@@ -522,8 +516,9 @@ __device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNo
 
 // Fused PCG update (femfunc.cpp:66-81): alpha from the per-patch p'q
 // partials (every block reduces them in the same fixed order), finish the
-// interface nodes of q (and p), x += alpha p, r -= alpha q, z = invD r, and
-// per-block r'z, r'r partials for the next iteration's stop test.  A
+// interface nodes of q (and p = invD r + beta p), x += alpha p,
+// r -= alpha q, and per-block r'z, r'r partials (z = invD r) for the next
+// iteration's stop test; z itself is formed again where it is needed.  A
 // thread's first node is loaded before the partials are reduced: those
 // loads do not depend on alpha, so their latency overlaps the reduction.
 template <typename T, int DIM>
@@ -577,18 +572,17 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
     a.prr[blockIdx.x] = rr;
   }
 }
-// PCG initialisation: z = invD r and the r'z, r'r partials that the first
+// PCG initialisation: the r'z, r'r partials (z = invD r) that the first
 // fused iteration (parity 0) reads from slot 1.
 template <typename T>
-__global__ __launch_bounds__(UPT) void k_pcg_init(int64_t n, const T *r, T *z, const T *invD,
+__global__ __launch_bounds__(UPT) void k_pcg_init(int64_t n, const T *r, const T *invD,
                                                   double *prz, double *prr) {
   __shared__ double s_red[UPT / 64];
   double rz = 0.0, rr = 0.0;
   const int64_t stride = int64_t(gridDim.x) * UPT;
   for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < n; i += stride) {
     const T rv = r[i];
-    const T zv = invD[i] * rv;
-    z[i] = zv;
+    const T zv = mul_rn(invD[i], rv);
     rz += double(rv) * double(zv);
     rr += double(rv) * double(rv);
   }
@@ -696,20 +690,19 @@ __global__ __launch_bounds__(NT) void k_flush_x(int64_t n, double *x64, float *x
   }
 }
 
-// Residual replacement: r32 = r64 (the FP64 residual f - K x64), z32 =
-// invD r32, and the r'z, r'r partials the next fused iteration (parity 0)
-// reads from slot 1.
+// Residual replacement: r32 = r64 (the FP64 residual f - K x64) and the
+// r'z, r'r partials (z = invD r32) the next fused iteration (parity 0) reads
+// from slot 1.
 __global__ __launch_bounds__(UPT) void k_pcg_replace(int64_t n, const double *r64, float *r32,
-                                                     float *z32, const float *invD, double *prz,
+                                                     const float *invD, double *prz,
                                                      double *prr) {
   __shared__ double s_red[UPT / 64];
   double rz = 0.0, rr = 0.0;
   const int64_t stride = int64_t(gridDim.x) * UPT;
   for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < n; i += stride) {
     const float rv = float(r64[i]);
-    const float zv = invD[i] * rv;
+    const float zv = mul_rn(invD[i], rv);
     r32[i] = rv;
-    z32[i] = zv;
     rz += double(rv) * double(zv);
     rr += double(rv) * double(rv);
   }
@@ -1189,20 +1182,21 @@ void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T
 }
 
 template <typename T>
-void launch_pcg_init(const DevMesh &dm, const T *r, T *z, const T *invD, CgScalars *sc,
+void launch_pcg_init(const DevMesh &dm, const T *r, const T *invD, CgScalars *sc,
                      double *part_node, hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
   double *prz = part_node + 2 * NODE_PARTS, *prr = part_node + 3 * NODE_PARTS;  // slot 1
-  hipLaunchKernelGGL((k_pcg_init<T>), dim3(NODE_PARTS), dim3(UPT), 0, s, n, r, z, invD, prz, prr);
+  hipLaunchKernelGGL((k_pcg_init<T>), dim3(NODE_PARTS), dim3(UPT), 0, s, n, r, invD, prz, prr);
   hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, prz, prr, NODE_PARTS);
 }
 
 template <typename T>
-void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial, const T *invD,
+void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, const T *invD,
                      OpShift<T> sh, CgScalars *sc, double *part_patch, double *part_node,
                      int parity, hipStream_t s) {
   PatchArgs<T> a = patch_args<T>(dm);
-  a.x = z;
+  a.x = r;
+  a.invD = invD;
   a.p = p;
   a.y = q;
   a.partial = partial;
@@ -1225,7 +1219,6 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *z, T *p, T *q, T *partial
   u.p = p;
   u.x = x;
   u.r = r;
-  u.z = z;
   u.invD = invD;
   u.sh = sh;
   u.sc = sc;
@@ -1257,10 +1250,10 @@ void launch_flush_x(const DevMesh &dm, double *x64, float *x32, hipStream_t s) {
   hipLaunchKernelGGL(k_flush_x, dim3(grid_for(n)), dim3(NT), 0, s, n, x64, x32);
 }
 
-void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, float *z32,
-                        const float *invD, double *part_node, hipStream_t s) {
+void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, const float *invD,
+                        double *part_node, hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
-  hipLaunchKernelGGL(k_pcg_replace, dim3(NODE_PARTS), dim3(UPT), 0, s, n, r64, r32, z32, invD,
+  hipLaunchKernelGGL(k_pcg_replace, dim3(NODE_PARTS), dim3(UPT), 0, s, n, r64, r32, invD,
                      part_node + 2 * NODE_PARTS, part_node + 3 * NODE_PARTS);
 }
 
@@ -1450,9 +1443,9 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_diag<T>(const DevMesh &, T *, T *, hipStream_t);                        \
   template void launch_inv_diag<T>(const DevMesh &, const T *, T, T, const T *, T *,           \
                                    hipStream_t);                                               \
-  template void launch_pcg_init<T>(const DevMesh &, const T *, T *, const T *, CgScalars *,    \
+  template void launch_pcg_init<T>(const DevMesh &, const T *, const T *, CgScalars *,         \
                                    double *, hipStream_t);                                     \
-  template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, T *, const T *,   \
+  template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, const T *,        \
                                    OpShift<T>, CgScalars *, double *, double *, int,           \
                                    hipStream_t);                                               \
   template void launch_fill<T>(T *, int64_t, T, hipStream_t);                                  \
