@@ -650,6 +650,8 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
       (st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)) ||
       (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb)))
     return st;
+  femgpu::launch_pcg_apply_x<T>(ctx->dm, w.x, w.p, sc, ctx->stream);
+  HIPCHK(hipGetLastError());
   *iters = ctx->h_sc->iter_final;
   *done_code = ctx->h_sc->done;
   return FEMGPU_OK;
@@ -1740,6 +1742,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
       const int done = ctx->h_sc->done;
       const double rr_rec = ctx->h_sc->rr_final;
       iters = ctx->h_sc->iter_final;
+      femgpu::launch_pcg_apply_x<float>(ctx->dm, w.x, w.p, sc, ctx->stream);
       femgpu::launch_flush_x(ctx->dm, x64, w.x, ctx->stream);
       // a breakdown right after a replacement that kept p: restart from the
       // FP64 residual (no block of the failing update wrote x or r)

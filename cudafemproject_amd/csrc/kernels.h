@@ -60,6 +60,10 @@ struct CgScalars {
   double g_final;      // last gamma seen by the stop test
   double rr_final;     // last r'r seen by the stop test
   double pq;           // last p'q (diagnostic)
+  // standard PCG in FP64: the step length alpha of the last update whose
+  // x += alpha p has not been applied yet (the next iteration's patch kernel
+  // applies it, launch_pcg_apply_x after a stop; 0 when none is pending)
+  double x_pend;
   double rr;           // r'r of the FP64 refinement residual
   double scale;        // sqrt(rr) (refinement scaling)
   int it[2];           // updates done before iteration j
@@ -112,6 +116,10 @@ template <typename T>
 void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, const T *invD,
                      OpShift<T> sh, CgScalars *sc, double *part_patch, double *part_node,
                      int parity, hipStream_t s);
+// After the last replay of a standard PCG run: the x += alpha p of its
+// last update, which the next iteration would have applied (sc->x_pend).
+template <typename T>
+void launch_pcg_apply_x(const DevMesh &dm, T *x, const T *p, CgScalars *sc, hipStream_t s);
 constexpr int NODE_PARTS = 512;  // blocks of the node-parallel PCG kernels
 static_assert(NODE_PARTS % 256 == 0, "the patch kernels read the partials in chunks of 256");
 

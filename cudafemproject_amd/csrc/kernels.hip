@@ -50,6 +50,11 @@ using namespace patch;
 // by every patch that holds it, from the previous launch's slots (the first
 // patch stores the result).
 
+// FP64 PCG: x += alpha p in the next iteration's patch kernel (k_ebe_patch
+// MODE 1); FP32: in the update kernel.
+template <typename T>
+constexpr bool PCG_DEFER_X = sizeof(T) == 8;
+
 template <typename T>
 struct PatchArgs {
   const PatchInfo *patches;
@@ -68,6 +73,7 @@ struct PatchArgs {
   int64_t n_owned;
   const T *x;   // MODE 0: x; MODE 1: r (z = invD r is formed here)
   const T *invD;      // MODE 1: Jacobi preconditioner
+  T *xs;              // MODE 1: the PCG iterate (x += alpha p of the previous update)
   T *p;         // MODE 1: search direction (old in, new out for owned nodes)
   T *y;         // owned-node output
   T *partial;   // interface slots
@@ -133,7 +139,7 @@ void k_ebe_patch(PatchArgs<T> a) {
   T *s_res = reinterpret_cast<T *>(lds + L.off_res);  // one component of the entries
   double *s_red = reinterpret_cast<double *>(lds + L.off_red);
 
-  T beta = T(0);
+  T beta = T(0), xstep = T(0);
   if constexpr (MODE == 1) {
     CgScalars *sc = a.sc;
     if (sc->done) return;
@@ -197,6 +203,7 @@ void k_ebe_patch(PatchArgs<T> a) {
     }
     if (stop) return;
     beta = T(bt);
+    xstep = T(sc->x_pend);
   }
   // this thread's node id and run end from the patch node tables, at a
   // fixed stride: they do not wait for the PatchInfo load, so both are in
@@ -273,11 +280,22 @@ void k_ebe_patch(PatchArgs<T> a) {
 #pragma unroll
       for (int c = 0; c < DIM; ++c) xin[c] = a.x[go + c];
     } else if constexpr (MODE == 1) {
+      T pv[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) xin[c] = pcg_dir(a.invD[go + c], a.x[go + c], beta, a.p[go + c]);
+      for (int c = 0; c < DIM; ++c) pv[c] = a.p[go + c];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) xin[c] = pcg_dir(a.invD[go + c], a.x[go + c], beta, pv[c]);
       if (own) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) a.p[go + c] = xin[c];
+      }
+      // FP64: the previous update's x += alpha p, by the node's first patch
+      // (the update kernel then neither reads nor writes x).  Two rounds of
+      // FP64 workgroups per CU hide these accesses; FP32 workgroups all
+      // stage their nodes at once, bandwidth-bound, and keep x in the update.
+      if (PCG_DEFER_X<T> && first) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) a.xs[go + c] += xstep * pv[c];
       }
     } else if constexpr (MODE == 3) {
       // the order of k_dyn_predict: (d + dt v) + c1 a
@@ -453,7 +471,7 @@ struct UpdArgs {
   const int32_t *slot_start;
   const T *partial;
   const T *q;
-  T *p, *x, *r;
+  T *p, *r, *x;
   const T *invD;
   OpShift<T> sh;
   CgScalars *sc;
@@ -467,7 +485,7 @@ constexpr int UPT = 512;  // threads per block of the node-parallel PCG kernels
 // One node of the PCG update, loaded before alpha is known.
 template <typename T, int DIM>
 struct UpdNode {
-  T q[DIM], p[DIM], x[DIM], r[DIM], d[DIM];
+  T q[DIM], p[DIM], r[DIM], d[DIM], x[DIM];
 };
 
 template <typename T, int DIM>
@@ -485,7 +503,7 @@ __device__ __forceinline__ void upd_load(const UpdArgs<T> &a, int64_t i, UpdNode
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
     n.p[c] = a.p[o + c];
-    n.x[c] = a.x[o + c];
+    if constexpr (!PCG_DEFER_X<T>) n.x[c] = a.x[o + c];
     n.r[c] = a.r[o + c];
     n.d[c] = a.invD[o + c];
   }
@@ -505,7 +523,7 @@ __device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNo
   }
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
-    a.x[o + c] = n.x[c] + alpha * n.p[c];
+    if constexpr (!PCG_DEFER_X<T>) a.x[o + c] = n.x[c] + alpha * n.p[c];
     const T rv = n.r[c] - alpha * n.q[c];
     a.r[o + c] = rv;
     const T zv = mul_rn(n.d[c], rv);
@@ -516,9 +534,11 @@ __device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNo
 
 // Fused PCG update (femfunc.cpp:66-81): alpha from the per-patch p'q
 // partials (every block reduces them in the same fixed order), finish the
-// interface nodes of q (and p = invD r + beta p), x += alpha p,
-// r -= alpha q, and per-block r'z, r'r partials (z = invD r) for the next
-// iteration's stop test; z itself is formed again where it is needed.  A
+// interface nodes of q (and p = invD r + beta p), x += alpha p (FP64: left
+// to the next iteration's patch kernel, which loads p anyway, with alpha in
+// sc->x_pend), r -= alpha q, and per-block r'z, r'r partials (z = invD r)
+// for the next iteration's stop test; z itself is formed again where it is
+// needed.  A
 // thread's first node is loaded before the partials are reduced: those
 // loads do not depend on alpha, so their latency overlaps the reduction.
 template <typename T, int DIM>
@@ -550,14 +570,19 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   if (!(pq > 0.0) || !isfinite(alpha_d)) {
     // no block writes x or r: the state is the last completed iteration's,
     // as after a stop in the patch kernel's preamble at this parity
+    // (this launch's patch kernel already applied the previous x step)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->pq = pq;
+      sc->x_pend = 0.0;
       sc->done = isfinite(pq) ? 3 : 2;
       sc->par_final = a.parity;
     }
     return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq = pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->pq = pq;
+    if constexpr (PCG_DEFER_X<T>) sc->x_pend = alpha_d;
+  }
   const T alpha = T(alpha_d), beta = T(sc->b[a.parity]);
   double rz = 0.0, rr = 0.0;
   if (i0 < a.n_active) upd_finish<T, DIM>(a, i0, n0, alpha, beta, rz, rr);
@@ -572,6 +597,20 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
     a.prr[blockIdx.x] = rr;
   }
 }
+// The x += alpha p still pending after the last update of a stopped or
+// paused standard PCG (sc->x_pend; 0: none), then none pending.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pcg_apply_x(int64_t n, T *x, const T *p,
+                                                    const CgScalars *sc) {
+  const T al = T(sc->x_pend);
+  if (al == T(0)) return;
+  const int64_t stride = int64_t(gridDim.x) * NT;
+  for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) x[i] += al * p[i];
+}
+__global__ void k_clear_x_pend(CgScalars *sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc->x_pend = 0.0;
+}
+
 // PCG initialisation: the r'z, r'r partials (z = invD r) that the first
 // fused iteration (parity 0) reads from slot 1.
 template <typename T>
@@ -638,7 +677,7 @@ __global__ void k_reset_scalars(CgScalars *sc, int kind, double eps, int max_ite
     sc->g[0] = sc->g[1] = 1.0;
     sc->b[0] = sc->b[1] = 0.0;
     sc->al[0] = sc->al[1] = 1.0;
-    sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = sc->pause = 0.0;
+    sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = sc->pause = sc->x_pend = 0.0;
     sc->it[0] = sc->it[1] = -1;
     sc->iter_final = sc->par_final = 0;
     sc->eps = eps;
@@ -1197,6 +1236,7 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, cons
   PatchArgs<T> a = patch_args<T>(dm);
   a.x = r;
   a.invD = invD;
+  a.xs = x;
   a.p = p;
   a.y = q;
   a.partial = partial;
@@ -1217,8 +1257,8 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, cons
   u.partial = partial;
   u.q = q;
   u.p = p;
-  u.x = x;
   u.r = r;
+  u.x = x;
   u.invD = invD;
   u.sh = sh;
   u.sc = sc;
@@ -1231,6 +1271,13 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, cons
     hipLaunchKernelGGL((k_pcg_update<T, 3>), dim3(NODE_PARTS), dim3(UPT), 0, s, u);
   else
     hipLaunchKernelGGL((k_pcg_update<T, 2>), dim3(NODE_PARTS), dim3(UPT), 0, s, u);
+}
+
+template <typename T>
+void launch_pcg_apply_x(const DevMesh &dm, T *x, const T *p, CgScalars *sc, hipStream_t s) {
+  const int64_t n = dm.n_active * dm.dim;
+  hipLaunchKernelGGL((k_pcg_apply_x<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, x, p, sc);
+  hipLaunchKernelGGL(k_clear_x_pend, dim3(1), dim3(64), 0, s, sc);
 }
 
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
@@ -1445,6 +1492,8 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
                                    hipStream_t);                                               \
   template void launch_pcg_init<T>(const DevMesh &, const T *, const T *, CgScalars *,         \
                                    double *, hipStream_t);                                     \
+  template void launch_pcg_apply_x<T>(const DevMesh &, T *, const T *, CgScalars *,            \
+                                      hipStream_t);                                            \
   template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, const T *,        \
                                    OpShift<T>, CgScalars *, double *, double *, int,           \
                                    hipStream_t);                                               \
