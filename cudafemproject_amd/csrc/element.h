@@ -61,6 +61,16 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
+// Workgroup barrier that orders LDS accesses only: global loads and stores
+// issued before it stay in flight (each load is waited for at its first
+// use).  For kernels whose data guarded by the barrier is in LDS and that
+// keep loads for later work in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Two block-wide sums with one pair of barriers; sh holds 2 * BT / 64 doubles.
 template <int BT>
 __device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {

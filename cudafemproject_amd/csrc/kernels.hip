@@ -222,10 +222,15 @@ void k_ebe_patch(PatchArgs<T> a) {
   const T cK = MODE == 1 ? a.sh.cK : T(1);
   const T cM = MODE == 1 ? a.sh.cM : T(0);
 
-  // Element records of this thread, loaded first so their latency overlaps
-  // the node staging.  They lie at a fixed stride per patch (zero padding
-  // past the patch's elements), so they too are in flight before the
-  // PatchInfo arrives.
+  // Element records of this thread.  They lie at a fixed stride per patch,
+  // so they are in flight before the PatchInfo arrives; indices are clamped
+  // to the stride in the matvec and the PCG step (records past the patch's
+  // elements are never decoded), so the loads are unconditional.  These two
+  // kernels load the first
+  // element's record here and the others after the node loads: every
+  // workgroup stages its nodes at the same time, bandwidth-bound, and the
+  // later records arrive while the first element is computed.
+  constexpr bool LATE_REC = MODE == 0 || MODE == 1;
   uint3 rec[EPT];
   T esc[EPT];
   // per-patch bases (uniform) and small unsigned thread offsets: scalar
@@ -233,9 +238,12 @@ void k_ebe_patch(PatchArgs<T> a) {
   const uint3 *prec = reinterpret_cast<const uint3 *>(a.erec_s) + size_t(pid) * unsigned(a.emax);
   const T *pesc = a.escale ? a.escale + P.elem_start : nullptr;
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) {
+  for (int e = 0; e < (LATE_REC ? 1 : EPT); ++e) {
     const int i = threadIdx.x + e * PT;
-    rec[e] = i < a.emax ? prec[unsigned(i)] : make_uint3(0u, 0u, 0u);
+    if constexpr (LATE_REC)
+      rec[e] = prec[unsigned(min(i, a.emax - 1))];
+    else
+      rec[e] = i < a.emax ? prec[unsigned(i)] : make_uint3(0u, 0u, 0u);
   }
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -346,7 +354,15 @@ void k_ebe_patch(PatchArgs<T> a) {
   // mass term of p'(cK K + cM M) p)
   unsigned fmf = fm | (first ? 0x10u : 0u);
   asm volatile("" : "+v"(fmf));  // opaque: not re-derived from its sources later
-  __syncthreads();
+  if constexpr (LATE_REC) {
+#pragma unroll
+    for (int e = 1; e < EPT; ++e)
+      rec[e] = prec[unsigned(min(int(threadIdx.x) + e * PT, a.emax - 1))];
+    // (the staged data is in LDS; the records stay in flight)
+    lds_barrier();
+  } else {
+    __syncthreads();
+  }
 
   // Phases B and C: element forces, node-sorted LDS assembly, node sums.
   T acc[DIM];
@@ -542,8 +558,9 @@ __device__ __forceinline__ void upd_finish(const UpdArgs<T> &a, int64_t i, UpdNo
 // sc->x_pend), r -= alpha q, and per-block r'z, r'r partials (z = invD r)
 // for the next iteration's stop test; z itself is formed again where it is
 // needed.  A
-// thread's first node is loaded before the partials are reduced: those
-// loads do not depend on alpha, so their latency overlaps the reduction.
+// thread's first two nodes are loaded before the partials are reduced:
+// those loads do not depend on alpha, so their latency overlaps the
+// reduction.
 template <typename T, int DIM>
 __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   __shared__ double s_red[2 * UPT / 64];
@@ -551,7 +568,7 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   const int64_t stride = int64_t(gridDim.x) * UPT;
   const int64_t i0 = int64_t(blockIdx.x) * UPT + threadIdx.x;
   // The per-patch p'q partials are loaded first (up to PQ_REG per thread
-  // into registers), then this thread's first node: the partials return
+  // into registers), then this thread's first two nodes: the partials return
   // first and their reduction does not wait for the node loads.
   constexpr int PQ_REG = 8;
   double pqv[PQ_REG];
@@ -560,8 +577,14 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
     const int i = threadIdx.x + j * UPT;
     pqv[j] = i < a.np ? a.ppq[i] : 0.0;
   }
-  UpdNode<T, DIM> n0;
+  // A thread's first two nodes are loaded before the reduction: with one
+  // node per thread short of covering the mesh, the second node's load
+  // chain (an interface node's slot offsets, then its slots) would
+  // otherwise start only after the first node is finished.
+  const int64_t i1 = i0 + stride;
+  UpdNode<T, DIM> n0, n1;
   if (i0 < a.n_active) upd_load<T, DIM>(a, i0, n0);
+  if (i1 < a.n_active) upd_load<T, DIM>(a, i1, n1);
   double pq = 0.0;
 #pragma unroll
   for (int j = 0; j < PQ_REG; ++j) pq += pqv[j];
@@ -589,7 +612,8 @@ __global__ __launch_bounds__(UPT) void k_pcg_update(UpdArgs<T> a) {
   const T alpha = T(alpha_d), beta = T(sc->b[a.parity]);
   double rz = 0.0, rr = 0.0;
   if (i0 < a.n_active) upd_finish<T, DIM>(a, i0, n0, alpha, beta, rz, rr);
-  for (int64_t i = i0 + stride; i < a.n_active; i += stride) {
+  if (i1 < a.n_active) upd_finish<T, DIM>(a, i1, n1, alpha, beta, rz, rr);
+  for (int64_t i = i1 + stride; i < a.n_active; i += stride) {
     UpdNode<T, DIM> n;
     upd_load<T, DIM>(a, i, n);
     upd_finish<T, DIM>(a, i, n, alpha, beta, rz, rr);
