@@ -392,6 +392,77 @@ def bench_batch(args, F):
     ctx.close()
 
 
+def bench_contraction(args, F):
+    """The element contraction of the batch workload's triangles, F_e = K_e U_e
+    for its four load cases, on the fly (the solver's form) against K_e and
+    v_mfma_f32_4x4x1f32 (libfemgpu_probe.so); same forces, event-timed.  The
+    timed output is each load case's element energy u' F (16 B per element);
+    with the full forces written (96 B per element) the launches are
+    store-bound and are reported beside it."""
+    import torch
+    from cudafemproject_amd import contraction_probe as CP
+    from cudafemproject_amd.problem import plate_batch
+    insts = plate_batch(512, seed=2024)
+    coords, elems = CP.flat_plates(insts)
+    m, n = len(elems), len(coords)
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((n, CP.N_CASES, 2)).astype(np.float32)
+    E, nu = 2e7, 0.3
+    lam, mu = E * nu / (1 - nu * nu), E / (2 * (1 + nu))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    te, tc, tu = dev(elems), dev(coords), dev(u)
+    st = torch.cuda.current_stream()
+    reps = max(args.matvec_reps, 20)
+
+    def timed(v, out):
+        CP.contract(v, te, tc, tu, lam, mu, out, reps=5)
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            CP.contract(v, te, tc, tu, lam, mu, out, reps=reps)
+            e1.record(st)
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) / reps * 1e3
+            best = us if best is None else min(best, us)
+        return best
+
+    # energies u_case' F_case per element (16 B out per element): the contraction
+    g = [torch.empty((m, CP.N_CASES), device="cuda") for _ in range(2)]
+    t = {v: timed(v, g[v]) for v in (0, 1)}
+    # full forces (96 B out per element): store-bound
+    f = [torch.empty((m, CP.N_CASES, 6), device="cuda") for _ in range(2)]
+    tf = {v: timed(v, f[v]) for v in (0, 1)}
+    a, b = f[0].cpu().numpy(), f[1].cpu().numpy()
+    diff = float((np.abs(a - b) / np.abs(a).max(axis=2, keepdims=True)).max())
+    log(f"{m} triangles x {CP.N_CASES} load cases, energies out: on the fly {t[0]:.2f} us, "
+        f"K_e + MFMA {t[1]:.2f} us; forces out: {tf[0]:.2f} / {tf[1]:.2f} us; "
+        f"max relative force difference {diff:.2e}")
+    # bytes a launch must move: connectivity, coordinates and load vectors
+    # once each, energies written
+    byt = 12 * m + 8 * n + 4 * CP.N_CASES * 2 * n + 4 * CP.N_CASES * m
+    line = {
+        "metric": "stacked-load-case element contraction F_e = K_e U_e (batch plates' triangles, "
+                  "4 load cases): time per launch",
+        "value": t[0], "unit": "us", "n_gpus": 1, "steps": 3, "warmup": 5,
+        "ms_per_step": t[0] * 1e-3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "the 512 plates of the batch workload (seed 2024) as one mesh, "
+                               "random load vectors", "n_elems": m, "n_nodes": n,
+                   "n_cases": CP.N_CASES},
+        "on_the_fly_us": t[0], "ke_mfma_us": t[1], "mfma_over_on_the_fly": t[1] / t[0],
+        "forces_out_on_the_fly_us": tf[0], "forces_out_ke_mfma_us": tf[1],
+        "max_rel_force_diff": diff,
+        "roofline": {"bound": "hbm", "achieved": byt / (t[0] * 1e-6) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": byt / (t[0] * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_contract_valu", "bytes_per_launch": byt},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def traffic_probe(args, F):
     """The command profiled for the HBM traffic figures: 1 + 5 FP32 cube
     matvecs, then 3 device copies of a 256 MiB buffer as the counters'
@@ -416,7 +487,8 @@ def traffic_probe(args, F):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["cube", "column", "implicit", "cantilever", "batch"],
+    ap.add_argument("--workload", choices=["cube", "column", "implicit", "cantilever", "batch",
+                                           "contraction"],
                     default="cube",
                     help="cube: static solve (default, headline); column: 20,000-step explicit "
                          "dynamics; implicit: 200 implicit Newmark steps on the same column; "
@@ -457,6 +529,8 @@ def main():
         return bench_cantilever(args, F)
     if args.workload == "batch":
         return bench_batch(args, F)
+    if args.workload == "contraction":
+        return bench_contraction(args, F)
 
     t0 = time.perf_counter()
     pb = cube_problem(args.n, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)

@@ -114,8 +114,24 @@ def ensure_fem_main() -> str:
     return FEM_MAIN
 
 
+PROBE_LIB = os.path.join(PKG, "libfemgpu_probe.so")
+
+
+def build_probe(out: str = PROBE_LIB) -> str:
+    """Compile libfemgpu_probe.so: the measurement kernels of the
+    stacked-load-case element contraction (on the fly against K_e with the
+    matrix instruction), not part of the solver library."""
+    hipcc = _hipcc()
+    tmp = out + ".tmp"
+    _run([hipcc, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-shared", "-o", tmp,
+          os.path.join(CSRC, "contraction_probe.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all():
     build_library()
+    build_probe()
     build_fem_main()
 
 
