@@ -139,7 +139,9 @@ def omega_max(ctx, pb, m, iters=60):
 
 def bench_column(args, F):
     """Explicit GN22 (central difference, lumped mass) on the 16x16x256 column,
-    Ricker 50 Hz top pressure, dt = 0.5 * 2/omega_max, 20,000 steps."""
+    Ricker 50 Hz top pressure, dt = 0.5 * 2/omega_max, 20,000 steps; the patch
+    tuning of each precision from femgpu_autotune_dyn (--no-autotune: the
+    default)."""
     from cudafemproject_amd.problem import column_problem
     pb = column_problem(16, 16, 256, 0.1, 0.1, 1.6, E=2e7, nu=0.25, pressure=1.0,
                         convention=F.LOADS_PHYSICAL)
@@ -152,10 +154,15 @@ def bench_column(args, F):
     log(f"column: {pb.n_elems} elems, {pb.n_dof} dof, omega_max {w:.4e}, dt {dt:.4e}, "
         f"{nsteps} steps")
     res = {}
+    tuning = {}
     for name, prec in (("fp64", F.FP64), ("fp32", F.FP32)):
         kw = dict(rho=rho, beta1=0.5, beta2=0.0, dt=dt, endtime=(nsteps + 0.5) * dt,
                   wavelet_type=F.WAVELET_RICKER, freq=50.0, timeshift=0.0, ampl=1e3,
                   precision=prec, mass_convention=F.MASS_PHYSICAL)
+        if not args.no_autotune:
+            # the patch tuning of this stepper and precision (femgpu_autotune_dyn)
+            tuning[name] = ctx.autotune_dyn(**{k: v for k, v in kw.items() if k != "endtime"})
+            log(f"{name}: autotuned patches {tuning[name]}")
         for _ in range(args.warmup):
             ctx.dyn_run(**dict(kw, endtime=200 * dt))
         runs = [ctx.dyn_run(**kw) for _ in range(args.steps)]
@@ -181,6 +188,7 @@ def bench_column(args, F):
                                "dt = 0.5 * 2/omega_max", "n_elems": pb.n_elems, "n_dof": N,
                    "dt": dt, "time_steps": n64},
         "us_per_step_fp32": us32, "steps_per_s_fp64": 1e6 / us64, "steps_per_s_fp32": 1e6 / us32,
+        "patch_tuning": tuning or "default (256 threads, 768 elements)",
         "fp32_vs_fp64_rel_displacement_diff": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
@@ -206,11 +214,15 @@ def bench_implicit(args, F):
     dt = 4.0 * 0.5 * 2.0 / omega_max(ctx, pb, lumped_mass(pb, rho))
     nsteps = 200
     res = {}
+    tuning = {}
     for name, prec in (("fp64", F.FP64), ("fp32", F.FP32)):
         kw = dict(rho=rho, beta1=0.5, beta2=0.5, dt=dt, endtime=(nsteps + 0.5) * dt,
                   wavelet_type=F.WAVELET_RICKER, freq=50.0, timeshift=0.0, ampl=1e3,
                   precision=prec, mass_convention=F.MASS_PHYSICAL,
                   implicit_mode=F.IMPLICIT_PHYSICAL, pcg_eps=1e-8)
+        if not args.no_autotune:
+            tuning[name] = ctx.autotune_dyn(**{k: v for k, v in kw.items() if k != "endtime"})
+            log(f"implicit {name}: autotuned patches {tuning[name]}")
         for _ in range(args.warmup):
             ctx.dyn_run(**dict(kw, endtime=20.5 * dt))
         runs = [ctx.dyn_run(**kw) for _ in range(args.steps)]
@@ -234,6 +246,7 @@ def bench_implicit(args, F):
         "pcg_iterations_per_step_fp64": it64 / n64, "pcg_iterations_per_step_fp32": it32 / n32,
         "us_per_step_fp32": ms32 * 1e3 / n32,
         "us_per_pcg_iteration_fp64": ms64 * 1e3 / it64,
+        "patch_tuning": tuning or "default (256 threads, 768 elements)",
         "fp32_vs_fp64_rel_displacement_diff": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
         "roofline": None, "cpu_baseline": None,
     }
@@ -498,6 +511,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the FP32 matvec + copy sequence profiled for HBM traffic")
     ap.add_argument("--fd-elems", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="column / implicit: keep the default patch tuning instead of "
+                         "femgpu_autotune_dyn")
     ap.add_argument("--no-persistent", action="store_true",
                     help="batch workload: skip the persistent-schedule comparison")
     ap.add_argument("--gpus", type=int, default=1)

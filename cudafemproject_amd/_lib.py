@@ -114,6 +114,8 @@ SIGNATURES = {
     "femgpu_compliance_grad": (C.c_int, [_P, C.POINTER(SolveOpts), _D, _D, _D, C.POINTER(Stats)]),
     "femgpu_default_dyn_opts": (C.c_int, [C.POINTER(DynOpts)]),
     "femgpu_dyn_run": (C.c_int, [_P, C.POINTER(DynOpts), _D, C.POINTER(DynStats)]),
+    "femgpu_autotune_dyn": (C.c_int, [_P, C.POINTER(DynOpts), C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -468,14 +470,12 @@ class Context:
         self._check(self.lib.femgpu_element_energy(self.h, _dp(u), _dp(e)), "element_energy")
         return e
 
-    def dyn_run(self, rho=2400.0, beta1=0.5, beta2=0.0, dt=1e-3, endtime=0.1,
-                wavelet_type=WAVELET_NONE, freq=15.0, timeshift=0.0, ampl=1.0,
-                precision=FP64, mass_convention=MASS_REFERENCE,
-                implicit_mode=IMPLICIT_REFERENCE, pcg_eps=1e-4, steps_per_graph=64,
-                damping_alpha=0.0, damping_beta=0.0, relax_reference=True, max_steps=1000000,
-                raise_on_error=True):
+    @staticmethod
+    def _dyn_opts(lib, rho, beta1, beta2, dt, endtime, wavelet_type, freq, timeshift, ampl,
+                  precision, mass_convention, implicit_mode, pcg_eps, steps_per_graph,
+                  damping_alpha, damping_beta, relax_reference, max_steps):
         o = DynOpts()
-        self.lib.femgpu_default_dyn_opts(C.byref(o))
+        lib.femgpu_default_dyn_opts(C.byref(o))
         o.damping_alpha, o.damping_beta = damping_alpha, damping_beta
         o.relax_reference, o.max_steps = int(bool(relax_reference)), max_steps
         o.rho, o.beta1, o.beta2, o.dt, o.endtime = rho, beta1, beta2, dt, endtime
@@ -483,6 +483,33 @@ class Context:
             wavelet_type, freq, timeshift, ampl)
         o.precision, o.mass_convention, o.implicit_mode = precision, mass_convention, implicit_mode
         o.pcg_eps, o.steps_per_graph = pcg_eps, steps_per_graph
+        return o
+
+    def autotune_dyn(self, n_steps=0, rho=2400.0, beta1=0.5, beta2=0.0, dt=1e-3,
+                     wavelet_type=WAVELET_NONE, freq=15.0, timeshift=0.0, ampl=1.0,
+                     precision=FP64, mass_convention=MASS_REFERENCE,
+                     implicit_mode=IMPLICIT_REFERENCE, pcg_eps=1e-4, steps_per_graph=64,
+                     damping_alpha=0.0, damping_beta=0.0) -> dict:
+        """femgpu_autotune_dyn: pick and apply the fastest patch tuning for
+        the time stepper these options select (timed on n_steps steps)."""
+        o = self._dyn_opts(self.lib, rho, beta1, beta2, dt, 0.0, wavelet_type, freq, timeshift,
+                           ampl, precision, mass_convention, implicit_mode, pcg_eps,
+                           steps_per_graph, damping_alpha, damping_beta, True, 1000000)
+        e, t, us = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._check(self.lib.femgpu_autotune_dyn(self.h, C.byref(o), int(n_steps), C.byref(e),
+                                                 C.byref(t), C.byref(us)), "autotune_dyn")
+        return {"elems_per_patch": e.value, "patch_threads": t.value, "us_per_step": us.value}
+
+    def dyn_run(self, rho=2400.0, beta1=0.5, beta2=0.0, dt=1e-3, endtime=0.1,
+                wavelet_type=WAVELET_NONE, freq=15.0, timeshift=0.0, ampl=1.0,
+                precision=FP64, mass_convention=MASS_REFERENCE,
+                implicit_mode=IMPLICIT_REFERENCE, pcg_eps=1e-4, steps_per_graph=64,
+                damping_alpha=0.0, damping_beta=0.0, relax_reference=True, max_steps=1000000,
+                raise_on_error=True):
+        o = self._dyn_opts(self.lib, rho, beta1, beta2, dt, endtime, wavelet_type, freq,
+                           timeshift, ampl, precision, mass_convention, implicit_mode, pcg_eps,
+                           steps_per_graph, damping_alpha, damping_beta, relax_reference,
+                           max_steps)
         u = np.zeros((self.n_nodes, self.dim))
         s = DynStats()
         st = self.lib.femgpu_dyn_run(self.h, C.byref(o), _dp(u), C.byref(s))

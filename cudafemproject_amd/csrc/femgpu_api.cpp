@@ -1121,8 +1121,9 @@ namespace {
 // Autotuner: candidate patch tunings (threads per patch, elements per patch),
 // the process-wide cache of choices per (mesh signature, arithmetic), and
 // the mesh signature (FNV-1a over dim, sizes, coordinates and connectivity).
-constexpr int TUNE_CAND[][2] = {{256, 768}, {256, 640}, {256, 512}, {512, 1536},
-                                {512, 1024}, {1024, 3072}, {1024, 2048}};
+constexpr int TUNE_CAND[][2] = {{256, 768}, {256, 640}, {256, 512}, {256, 384},
+                                {512, 1536}, {512, 1024}, {512, 768}, {1024, 3072},
+                                {1024, 2048}};
 constexpr int TUNE_N = int(sizeof(TUNE_CAND) / sizeof(TUNE_CAND[0]));
 
 struct TuneChoice {
@@ -1187,18 +1188,17 @@ int time_pcg_iteration(femgpu_ctx *ctx, int precision, int iters, double *ms) {
   return FEMGPU_OK;
 }
 
-}  // namespace
+// Device time per step of the time stepper with the current plan: the
+// second of two runs of `steps` steps (the first captures the step graphs).
+int time_dyn_step(femgpu_ctx *ctx, const femgpu_dyn_opts &opts, int steps, double *us);
 
-extern "C" {
-
-int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *patch_threads,
-                    double *ms_per_iteration) {
-  int st = need_mesh(ctx);
-  if (st) return st;
-  if (precision != FEMGPU_FP64 && precision != FEMGPU_FP32 && precision != FEMGPU_MIXED)
-    return fail(ctx, FEMGPU_E_ARG, "unknown precision");
-  const int arith = precision == FEMGPU_FP64 ? FEMGPU_FP64 : FEMGPU_FP32;
-  const auto key = std::make_pair(mesh_signature(ctx), arith);
+// The autotuners' common part: plans every candidate on host threads, times
+// each with `timer` (its time in *t; FEMGPU_E_BREAKDOWN or FEMGPU_E_MAXIT
+// skip the candidate), caches the fastest under `key` and leaves the
+// context planned with it as its tuning.  A cached choice is only re-planned.
+template <typename Timer>
+int tune_candidates(femgpu_ctx *ctx, std::pair<uint64_t, int> key, Timer timer,
+                    const char *unit, TuneChoice *out) {
   TuneChoice best{0, 0, 0.0};
   bool cached = false;
   {
@@ -1209,8 +1209,8 @@ int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *p
       cached = true;
     }
   }
+  int st;
   if (!cached) {
-    // plan every candidate on the host in parallel, then time each on the GPU
     std::vector<MeshPlan> plans(TUNE_N);
     std::vector<int> pst(TUNE_N, FEMGPU_OK);
     {
@@ -1226,20 +1226,19 @@ int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *p
     for (int i = 0; i < TUNE_N; ++i) {
       if (pst[i] != FEMGPU_OK) continue;
       if ((st = replan(ctx, std::move(plans[i]), TUNE_CAND[i][0]))) return st;
-      double ms = 0.0;
-      st = time_pcg_iteration(ctx, arith, 64, &ms);
-      if (st == FEMGPU_E_BREAKDOWN) continue;
+      double t = 0.0;
+      st = timer(&t);
+      if (st == FEMGPU_E_BREAKDOWN || st == FEMGPU_E_MAXIT) continue;
       if (st) return st;
       if (trace_enabled())
-        std::fprintf(stderr, "femgpu autotune: %d threads, %d elements per patch: %.4f ms/iteration\n",
-                     TUNE_CAND[i][0], TUNE_CAND[i][1], ms);
-      if (best.threads == 0 || ms < best.ms) best = {TUNE_CAND[i][0], TUNE_CAND[i][1], ms};
+        std::fprintf(stderr, "femgpu autotune: %d threads, %d elements per patch: %.4f %s\n",
+                     TUNE_CAND[i][0], TUNE_CAND[i][1], t, unit);
+      if (best.threads == 0 || t < best.ms) best = {TUNE_CAND[i][0], TUNE_CAND[i][1], t};
     }
     if (best.threads == 0) return fail(ctx, FEMGPU_E_STATE, "autotune: no candidate ran");
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune_cache[key] = best;
   }
-  // leave the context planned with the choice, and keep it as its tuning
   ctx->emax = best.elems;
   ctx->threads_req = best.threads;
   MeshPlan plan;
@@ -1248,9 +1247,54 @@ int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *p
                       best.threads, plan, msg)))
     return fail(ctx, st, msg);
   if ((st = replan(ctx, std::move(plan), best.threads))) return st;
+  *out = best;
+  return FEMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *patch_threads,
+                    double *ms_per_iteration) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (precision != FEMGPU_FP64 && precision != FEMGPU_FP32 && precision != FEMGPU_MIXED)
+    return fail(ctx, FEMGPU_E_ARG, "unknown precision");
+  const int arith = precision == FEMGPU_FP64 ? FEMGPU_FP64 : FEMGPU_FP32;
+  TuneChoice best{};
+  if ((st = tune_candidates(
+           ctx, std::make_pair(mesh_signature(ctx), arith),
+           [&](double *t) { return time_pcg_iteration(ctx, arith, 64, t); }, "ms/iteration",
+           &best)))
+    return st;
   if (elems_per_patch) *elems_per_patch = best.elems;
   if (patch_threads) *patch_threads = best.threads;
   if (ms_per_iteration) *ms_per_iteration = best.ms;
+  return FEMGPU_OK;
+}
+
+int femgpu_autotune_dyn(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, int n_steps,
+                        int *elems_per_patch, int *patch_threads, double *us_per_step) {
+  int st = need_mesh(ctx);
+  if (st) return st;
+  if (!opts) return FEMGPU_E_ARG;
+  if (!(opts->dt > 0) || !(opts->rho > 0)) return fail(ctx, FEMGPU_E_ARG, "dt and rho must be > 0");
+  if (opts->precision != FEMGPU_FP64 && opts->precision != FEMGPU_FP32)
+    return fail(ctx, FEMGPU_E_ARG, "dynamics precision must be FP64 or FP32");
+  if (n_steps < 0) return fail(ctx, FEMGPU_E_ARG, "n_steps must be >= 0");
+  const bool implicit = opts->beta2 != 0.0;
+  const int steps = n_steps ? n_steps : implicit ? 16 : 256;
+  // cached apart from the PCG choices: per stepper kind and precision
+  const int kind = 100 + (implicit ? 10 : 0) + opts->precision;
+  TuneChoice best{};
+  if ((st = tune_candidates(
+           ctx, std::make_pair(mesh_signature(ctx), kind),
+           [&](double *t) { return time_dyn_step(ctx, *opts, steps, t); }, "us/step", &best)))
+    return st;
+  if (elems_per_patch) *elems_per_patch = best.elems;
+  if (patch_threads) *patch_threads = best.threads;
+  if (us_per_step) *us_per_step = best.ms;
   return FEMGPU_OK;
 }
 
@@ -2388,3 +2432,20 @@ int femgpu_dyn_run(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, double *u_out,
 }
 
 }  // extern "C"
+
+namespace {
+
+int time_dyn_step(femgpu_ctx *ctx, const femgpu_dyn_opts &opts, int steps, double *us) {
+  femgpu_dyn_opts o = opts;
+  o.endtime = (steps + 0.5) * o.dt;  // a fixed number of steps (no relaxation)
+  femgpu_dyn_stats S{};
+  for (int rep = 0; rep < 2; ++rep) {
+    const int st = femgpu_dyn_run(ctx, &o, nullptr, &S);
+    if (st) return st;
+  }
+  if (S.steps < 1) return fail(ctx, FEMGPU_E_STATE, "autotune: no time step ran");
+  *us = S.run_ms * 1e3 / S.steps;
+  return FEMGPU_OK;
+}
+
+}  // namespace

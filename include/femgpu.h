@@ -361,6 +361,19 @@ typedef struct {
 int femgpu_default_dyn_opts(femgpu_dyn_opts *opts);
 int femgpu_dyn_run(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, double *u_out,
                    femgpu_dyn_stats *stats);
+/* Autotuner for the time steppers (no counterpart in the reference): as
+ * femgpu_autotune, but every candidate patch tuning is timed on n_steps time
+ * steps of femgpu_dyn_run with these options (explicit or implicit as
+ * opts->beta2 selects, endtime replaced; 0: 256 explicit or 16 implicit
+ * steps), the second of two runs, so that the choice fits the stepper's
+ * own kernels (the explicit step's patch kernel and launch shape differ
+ * from the PCG iteration's).  Re-plans the mesh with the fastest and keeps
+ * it as the context's tuning; cached for the process per mesh, stepper kind
+ * and precision.  The tuning runs are femgpu_dyn_run calls: receivers
+ * record them (femgpu_get_traces then returns the last one).  Outputs may
+ * be NULL. */
+int femgpu_autotune_dyn(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, int n_steps,
+                        int *elems_per_patch, int *patch_threads, double *us_per_step);
 
 /* Receivers (the reference's receiver traces for SEG-Y, writeDisplForSEGY,
  * VTKfile.cpp:72-87, called every segy_step steps in femfunc.cpp:226-243,
