@@ -59,6 +59,18 @@ def read_traffic(path):
     return None
 
 
+def traffic_profile(threads, elements):
+    """The committed PMC summary of the FP32 cube64 matvec at this patch
+    tuning (the default 256 / 768 keeps the plain name)."""
+    tag = "" if (threads, elements) == (256, 768) else f"_{threads}x{elements}"
+    return os.path.join(ROOT, "profiles", f"ebe_matvec_fp32_cube64{tag}_traffic.txt")
+
+
+def parse_patch(v):
+    t, e = v.lower().split("x")
+    return int(t), int(e)
+
+
 def cpu_baseline(pb, n_fp64_iters, cpu_iters):
     """The oracle's FP32 restatement of the reference's CPU path (element-local
     vectors, per-DOF sums in element order, single thread): setup + a bounded
@@ -476,14 +488,30 @@ def bench_contraction(args, F):
     print(json.dumps(line), flush=True)
 
 
+def bench_cube_mixed_only(args, F, ctx, tune, run):
+    """--only mixed (for profilers): the mixed solves and the FP32 matvec at
+    the mixed solve's tuning, in a process of their own."""
+    tune(F.MIXED, "mixed")
+    for _ in range(args.warmup):
+        run(F.MIXED, 1)
+    rmx = run(F.MIXED, args.steps)
+    ctx.sync()
+    tmx = np.mean([s["solve_ms"] for _, s in rmx])
+    mv32 = ctx.time_matvec(F.FP32, args.matvec_reps)
+    log(f"mixed only: {tmx:.2f} ms, {rmx[-1][1]['iterations']} it; matvec fp32 {mv32 * 1e3:.2f} us")
+
+
 def traffic_probe(args, F):
     """The command profiled for the HBM traffic figures: 1 + 5 FP32 cube
-    matvecs, then 3 device copies of a 256 MiB buffer as the counters'
-    calibration (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)."""
+    matvecs (at --patch, else the default tuning), then 3 device copies of a
+    256 MiB buffer as the counters' calibration (run under rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE)."""
     import torch
     from cudafemproject_amd.problem import cube_problem
     pb = cube_problem(args.n, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)
     ctx = F.Context(0)
+    if args.patch:
+        ctx.set_tuning(args.patch[1], args.patch[0])
     ctx.set_problem(pb)
     ctx.matvec_repeat(F.FP32, 1)
     ctx.sync()
@@ -512,8 +540,18 @@ def main():
                     help="only the FP32 matvec + copy sequence profiled for HBM traffic")
     ap.add_argument("--fd-elems", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true",
-                    help="column / implicit: keep the default patch tuning instead of "
-                         "femgpu_autotune_dyn")
+                    help="keep the default patch tuning instead of the autotuners "
+                         "(femgpu_autotune for the cube, femgpu_autotune_dyn for the "
+                         "column and implicit workloads)")
+    ap.add_argument("--patch", type=parse_patch, default=None, metavar="THREADSxELEMENTS",
+                    help="cube / traffic probe: this patch tuning for every precision")
+    ap.add_argument("--patch-fp64", type=parse_patch, default=None, metavar="THREADSxELEMENTS",
+                    help="cube: the FP64 solves' patch tuning")
+    ap.add_argument("--patch-mixed", type=parse_patch, default=None, metavar="THREADSxELEMENTS",
+                    help="cube: the mixed solves' patch tuning")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="cube, for profilers: one precision's solves and matvec in this "
+                         "process, no JSON line")
     ap.add_argument("--no-persistent", action="store_true",
                     help="batch workload: skip the persistent-schedule comparison")
     ap.add_argument("--gpus", type=int, default=1)
@@ -552,9 +590,26 @@ def main():
     pb = cube_problem(args.n, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)
     ctx = F.Context(0)
     ctx.set_problem(pb)
-    ctx.solve(F.FP64, tol=1e-3, tol_kind=F.TOL_REL_RES)  # builds diag, RHS, buffers
     log(f"setup {time.perf_counter() - t0:.2f} s: {pb.n_elems} elems, {pb.n_nodes} nodes, "
         f"{pb.n_dof} dof")
+    # Each precision runs at the patch tuning the library's autotuner picks
+    # for it (femgpu_autotune: timed PCG iterations per candidate, FP32 ones
+    # for the mixed solve); --no-autotune keeps the default; --patch,
+    # --patch-fp64, --patch-mixed set them.
+    tuning = {}
+    fixed = {"fp64": args.patch_fp64 or args.patch, "mixed": args.patch_mixed or args.patch}
+
+    def tune(prec, name):
+        if fixed[name]:
+            tuning[name] = {"patch_threads": fixed[name][0], "elems_per_patch": fixed[name][1]}
+            ctx.set_tuning(fixed[name][1], fixed[name][0])
+            ctx.set_problem(pb)
+            return
+        if args.no_autotune:
+            return
+        t = time.perf_counter()
+        tuning[name] = ctx.autotune(prec)
+        log(f"{name}: autotuned patches {tuning[name]} in {time.perf_counter() - t:.1f} s")
 
     def run(prec, k, rec=F.CG_STANDARD):
         out = []
@@ -563,52 +618,68 @@ def main():
             out.append((u, st))
         return out
 
+    b32 = bytes_per_matvec(pb.n_elems, pb.n_nodes, pb.n_dof, 4)
+    b64 = bytes_per_matvec(pb.n_elems, pb.n_nodes, pb.n_dof, 8)
+    # FP64: both recurrences and the FP64 matvec at the FP64 tuning
+    if args.only == "mixed":
+        return bench_cube_mixed_only(args, F, ctx, tune, run)
+    tune(F.FP64, "fp64")
     for _ in range(args.warmup):
         run(F.FP64, 1)
-        run(F.MIXED, 1)
     ctx.sync()
     r64 = run(F.FP64, args.steps)
     ctx.sync()
-    t_wall = time.perf_counter()
-    rmx = run(F.MIXED, args.steps)
-    ctx.sync()
-    t_wall = time.perf_counter() - t_wall
     t64 = np.mean([s["solve_ms"] for _, s in r64]) / 1e3
-    tmx = np.mean([s["solve_ms"] for _, s in rmx]) / 1e3
     u64, s64 = r64[-1]
-    umx, smx = rmx[-1]
-    err = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
     log(f"fp64: {t64 * 1e3:.2f} ms, {s64['iterations']} it, rel_res {s64['rel_res']:.2e}")
-    log(f"mixed: {tmx * 1e3:.2f} ms, {smx['iterations']} it ({smx['outer_iterations']} outer), "
-        f"rel_res {smx['rel_res']:.2e}, |u_mixed-u_fp64|/|u_fp64| {err:.2e}")
     # the same solves with the single-reduction recurrence (one launch per
     # iteration); the FP64 baseline of the speedup is the faster of the two
     t64s = tmxs = None
     it64s = itmxs = None
     if not args.no_single_reduction:
         run(F.FP64, 1, F.CG_SINGLE_REDUCTION)
-        run(F.MIXED, 1, F.CG_SINGLE_REDUCTION)
         r64s = run(F.FP64, args.steps, F.CG_SINGLE_REDUCTION)
-        rmxs = run(F.MIXED, args.steps, F.CG_SINGLE_REDUCTION)
         t64s = float(np.mean([s["solve_ms"] for _, s in r64s]) / 1e3)
+        it64s = r64s[-1][1]["iterations"]
+    mv64 = ctx.time_matvec(F.FP64, args.matvec_reps) / 1e3
+    if args.only == "fp64":
+        log(f"fp64 only: matvec fp64 {mv64 * 1e6:.2f} us")
+        return
+    # mixed: both recurrences and the FP32 matvec (the roofline kernel) at
+    # the mixed solve's tuning
+    tune(F.MIXED, "mixed")
+    for _ in range(args.warmup):
+        run(F.MIXED, 1)
+    ctx.sync()
+    t_wall = time.perf_counter()
+    rmx = run(F.MIXED, args.steps)
+    ctx.sync()
+    t_wall = time.perf_counter() - t_wall
+    tmx = np.mean([s["solve_ms"] for _, s in rmx]) / 1e3
+    umx, smx = rmx[-1]
+    err = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+    log(f"mixed: {tmx * 1e3:.2f} ms, {smx['iterations']} it ({smx['outer_iterations']} outer), "
+        f"rel_res {smx['rel_res']:.2e}, |u_mixed-u_fp64|/|u_fp64| {err:.2e}")
+    if not args.no_single_reduction:
+        run(F.MIXED, 1, F.CG_SINGLE_REDUCTION)
+        rmxs = run(F.MIXED, args.steps, F.CG_SINGLE_REDUCTION)
         tmxs = float(np.mean([s["solve_ms"] for _, s in rmxs]) / 1e3)
-        it64s, itmxs = r64s[-1][1]["iterations"], rmxs[-1][1]["iterations"]
+        itmxs = rmxs[-1][1]["iterations"]
         log(f"single-reduction recurrence: fp64 {t64s * 1e3:.2f} ms ({it64s} it), "
             f"mixed {tmxs * 1e3:.2f} ms ({itmxs} it)")
     t64_best = min(t64, t64s) if t64s else t64
-
     mv32 = ctx.time_matvec(F.FP32, args.matvec_reps) / 1e3
-    mv64 = ctx.time_matvec(F.FP64, args.matvec_reps) / 1e3
-    b32 = bytes_per_matvec(pb.n_elems, pb.n_nodes, pb.n_dof, 4)
-    b64 = bytes_per_matvec(pb.n_elems, pb.n_nodes, pb.n_dof, 8)
     log(f"matvec fp32 {mv32 * 1e6:.2f} us ({b32 / mv32 / 1e9:.0f} GB/s), "
         f"fp64 {mv64 * 1e6:.2f} us ({b64 / mv64 / 1e9:.0f} GB/s)")
 
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "ebe_matvec_fp32_cube64_traffic.txt"))
+    t32 = ((tuning["mixed"]["patch_threads"], tuning["mixed"]["elems_per_patch"])
+           if "mixed" in tuning else (256, 768))
+    traffic = read_traffic(traffic_profile(*t32))
     achieved = b32 / mv32 / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": "EbE matvec fp32 (k_ebe_patch + k_iface_gather), cube64",
+                "kernel": f"EbE matvec fp32 (k_ebe_patch + k_iface_gather), cube64, "
+                          f"{t32[0]}-thread patches of {t32[1]} elements",
                 "bytes_per_launch": b32}
 
     cgbs = None if args.no_copy_probe else copy_gbs()
@@ -632,7 +703,10 @@ def main():
                                "static PCG to rel-res 1e-8",
                    "n_elems": pb.n_elems, "n_nodes": pb.n_nodes, "n_dof": pb.n_dof,
                    "tol": 1e-8, "solver": "mixed (fp32 pcg in fp64 refinement)",
-                   "patch_tuning": {"threads": 256, "elements": 768}},
+                   "patch_tuning": ({k: {"threads": v["patch_threads"],
+                                         "elements": v["elems_per_patch"]}
+                                     for k, v in tuning.items()}
+                                    or {"threads": 256, "elements": 768})},
         "fp64_time_to_solve_s": float(t64),
         "speedup_mixed_vs_fp64": float(t64_best / tmx),
         # the same (standard) recurrence in both precisions

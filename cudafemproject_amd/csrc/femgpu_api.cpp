@@ -1163,9 +1163,11 @@ int replan(femgpu_ctx *ctx, MeshPlan &&plan, int threads) {
   return FEMGPU_OK;
 }
 
-// Device time per PCG iteration of the current plan: the second of two
-// runs of `iters` iterations (the first captures the iteration graphs) on a
-// unit right-hand side, stopping rule out of reach.
+// Device time per PCG iteration of the current plan on a unit right-hand
+// side, stopping rule out of reach: the difference between runs of 3 * iters
+// and iters iterations (each the faster of two, after a run that captures
+// the iteration graphs) over 2 * iters, so that the fixed cost of a solve
+// (initialisation, the last replay's wait) does not count.
 int time_pcg_iteration(femgpu_ctx *ctx, int precision, int iters, double *ms) {
   const size_t nd = size_t(ctx->nn) * ctx->dim;
   double *f;
@@ -1177,19 +1179,34 @@ int time_pcg_iteration(femgpu_ctx *ctx, int precision, int iters, double *ms) {
   o.precision = precision == FEMGPU_FP64 ? FEMGPU_FP64 : FEMGPU_FP32;
   o.tol_kind = FEMGPU_TOL_REL_RES;
   o.tol = 0.0;
-  o.max_iter = iters;
-  femgpu_stats S{};
+  // (a small problem's PCG may stop early, at a breakdown once its residual
+  // is at rounding level: the runs then count the iterations they did)
+  auto run = [&](int n, double *t, int *it) -> int {
+    o.max_iter = n;
+    femgpu_stats S{};
+    const int rs = solve_core(ctx, &o, "tune_f", nullptr, &S);
+    if (rs != FEMGPU_OK && rs != FEMGPU_E_MAXIT && rs != FEMGPU_E_BREAKDOWN) return rs;
+    if (S.iterations < 1) return fail(ctx, FEMGPU_E_BREAKDOWN, "autotune: PCG did not iterate");
+    *t = S.solve_ms;
+    *it = S.iterations;
+    return FEMGPU_OK;
+  };
+  double t1 = 0.0, t3 = 0.0, t;
+  int i1 = 0, i3 = 0, it;
+  if ((st = run(iters, &t, &it))) return st;
   for (int rep = 0; rep < 2; ++rep) {
-    st = solve_core(ctx, &o, "tune_f", nullptr, &S);
-    if (st != FEMGPU_OK && st != FEMGPU_E_MAXIT) return st;
+    if ((st = run(iters, &t, &it))) return st;
+    if (rep == 0 || t < t1) t1 = t, i1 = it;
+    if ((st = run(3 * iters, &t, &it))) return st;
+    if (rep == 0 || t < t3) t3 = t, i3 = it;
   }
-  if (S.iterations < iters / 2) return fail(ctx, FEMGPU_E_BREAKDOWN, "autotune: PCG stopped early");
-  *ms = S.solve_ms / S.iterations;
+  // without extra iterations in the longer run: the short run's average
+  *ms = i3 > i1 ? std::max(t3 - t1, 1e-9) / (i3 - i1) : t1 / i1;
   return FEMGPU_OK;
 }
 
-// Device time per step of the time stepper with the current plan: the
-// second of two runs of `steps` steps (the first captures the step graphs).
+// Device time per step of the time stepper with the current plan (see the
+// definition).
 int time_dyn_step(femgpu_ctx *ctx, const femgpu_dyn_opts &opts, int steps, double *us);
 
 // The autotuners' common part: plans every candidate on host threads, times
@@ -1265,7 +1282,7 @@ int femgpu_autotune(femgpu_ctx *ctx, int precision, int *elems_per_patch, int *p
   TuneChoice best{};
   if ((st = tune_candidates(
            ctx, std::make_pair(mesh_signature(ctx), arith),
-           [&](double *t) { return time_pcg_iteration(ctx, arith, 64, t); }, "ms/iteration",
+           [&](double *t) { return time_pcg_iteration(ctx, arith, 32, t); }, "ms/iteration",
            &best)))
     return st;
   if (elems_per_patch) *elems_per_patch = best.elems;
@@ -2437,14 +2454,31 @@ namespace {
 
 int time_dyn_step(femgpu_ctx *ctx, const femgpu_dyn_opts &opts, int steps, double *us) {
   femgpu_dyn_opts o = opts;
-  o.endtime = (steps + 0.5) * o.dt;  // a fixed number of steps (no relaxation)
-  femgpu_dyn_stats S{};
+  auto run = [&](int n, double *t, int *k) -> int {
+    o.endtime = (n + 0.5) * o.dt;  // a fixed number of steps (no relaxation)
+    femgpu_dyn_stats S{};
+    const int rs = femgpu_dyn_run(ctx, &o, nullptr, &S);
+    if (rs) return rs;
+    *t = S.run_ms;
+    *k = S.steps;
+    return FEMGPU_OK;
+  };
+  // the difference between runs of `steps` and steps / 4 steps (each the
+  // faster of two, after a run that captures the step graphs), so that the
+  // fixed cost of a run does not count
+  const int s1 = std::max(1, steps / 4);
+  double t1 = 0.0, t4 = 0.0, t;
+  int k1 = 0, k4 = 0, k;
+  int st;
+  if ((st = run(s1, &t, &k))) return st;
   for (int rep = 0; rep < 2; ++rep) {
-    const int st = femgpu_dyn_run(ctx, &o, nullptr, &S);
-    if (st) return st;
+    if ((st = run(s1, &t, &k))) return st;
+    if (rep == 0 || t < t1) t1 = t, k1 = k;
+    if ((st = run(steps, &t, &k))) return st;
+    if (rep == 0 || t < t4) t4 = t, k4 = k;
   }
-  if (S.steps < 1) return fail(ctx, FEMGPU_E_STATE, "autotune: no time step ran");
-  *us = S.run_ms * 1e3 / S.steps;
+  if (k4 <= k1) return fail(ctx, FEMGPU_E_STATE, "autotune: no time step ran");
+  *us = std::max(t4 - t1, 1e-9) * 1e3 / (k4 - k1);
   return FEMGPU_OK;
 }
 

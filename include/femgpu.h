@@ -137,12 +137,14 @@ int femgpu_set_mesh(femgpu_ctx *ctx, int dim, int64_t n_nodes, const double *nod
 int femgpu_set_tuning(femgpu_ctx *ctx, int elems_per_patch, int patch_threads);
 /* Autotuner (no counterpart in the reference, whose kernels use fixed
  * 256-thread blocks and one element per thread): for the current problem
- * (mesh, material, constraints set), times a short PCG run (64 iterations on
- * a unit right-hand side; FP64, or FP32 for FEMGPU_FP32 / FEMGPU_MIXED whose
- * inner iterations are FP32) under each candidate patch tuning (256, 512 or
- * 1024 threads per patch with 2 to 3 elements per thread; the candidates
- * are planned on host threads in parallel), re-plans the mesh with the
- * fastest and keeps it as the context's tuning (as femgpu_set_tuning).  The
+ * (mesh, material, constraints set), times the PCG iteration under each
+ * candidate patch tuning (256, 512 or 1024 threads per patch with 1.5 to 3
+ * elements per thread; the candidates are planned on host threads in
+ * parallel) -- the difference between runs of 96 and 32 iterations on a unit
+ * right-hand side, each the faster of two, so that a solve's fixed cost does
+ * not count; FP64, or FP32 for FEMGPU_FP32 / FEMGPU_MIXED whose inner
+ * iterations are FP32 -- re-plans the mesh with the fastest and keeps it as
+ * the context's tuning (as femgpu_set_tuning).  The
  * choice is cached for the process per mesh (hash of coordinates and
  * connectivity) and arithmetic, so a later call on the same mesh only
  * re-plans.  Outputs may be NULL. */
@@ -362,10 +364,11 @@ int femgpu_default_dyn_opts(femgpu_dyn_opts *opts);
 int femgpu_dyn_run(femgpu_ctx *ctx, const femgpu_dyn_opts *opts, double *u_out,
                    femgpu_dyn_stats *stats);
 /* Autotuner for the time steppers (no counterpart in the reference): as
- * femgpu_autotune, but every candidate patch tuning is timed on n_steps time
- * steps of femgpu_dyn_run with these options (explicit or implicit as
- * opts->beta2 selects, endtime replaced; 0: 256 explicit or 16 implicit
- * steps), the second of two runs, so that the choice fits the stepper's
+ * femgpu_autotune, but every candidate patch tuning is timed on time steps
+ * of femgpu_dyn_run with these options (explicit or implicit as opts->beta2
+ * selects, endtime replaced): the difference between runs of n_steps and
+ * n_steps / 4 steps (0: 256 explicit or 16 implicit steps), each the faster
+ * of two, so that the choice fits the stepper's
  * own kernels (the explicit step's patch kernel and launch shape differ
  * from the PCG iteration's).  Re-plans the mesh with the fastest and keeps
  * it as the context's tuning; cached for the process per mesh, stepper kind
