@@ -280,7 +280,12 @@ def bench_cantilever(args, F):
     rho = pb.rho
     ctx.compliance_grad(F.FP64, tol=1e-3)
     times = {}
+    tuning = {}
     for name, prec in (("fp64", F.FP64), ("mixed", F.MIXED)):
+        if not args.no_autotune:
+            # each precision at the patch tuning femgpu_autotune picks for it
+            tuning[name] = ctx.autotune(prec)
+            log(f"{name}: autotuned patches {tuning[name]}")
         for _ in range(args.warmup):
             ctx.compliance_grad(prec, tol=1e-8)
         t = []
@@ -333,6 +338,7 @@ def bench_cantilever(args, F):
                                "seed 5, E_e = 2e7 rho_e^3, nu 0.3, x=0 clamped, tip pressure",
                    "n_elems": pb.n_elems, "n_dof": pb.n_dof, "tol": 1e-8},
         "fp64_s": t64, "iterations_fp64": s64["iterations"], "iterations_mixed": smx["iterations"],
+        "patch_tuning": tuning or "default (256 threads, 768 elements)",
         "mixed_vs_fp64_grad_rel_l2": float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64)),
         "fd_elements": int(len(sample)), "fd_h": h,
         "fd_max_abs_err_over_max_fd": fd_rel_max, "fd_median_rel_err": fd_rel_med,
