@@ -628,10 +628,16 @@ int replay_until(femgpu_ctx *ctx, const Replay &exec, CgScalars *sc, int block, 
 inline int tail_block_of(int block) { return std::max(2, (block / 4) & ~1); }
 
 // Runs PCG until the stop flag is set.  r holds the initial residual; x and p
-// are cleared here.
+// are cleared here.  first > 0 (a caller that knows about how many iterations
+// the solve takes, such as a time stepper from its previous steps): one
+// replay of `first` iterations (rounded up to even) and the host looks at
+// the stop flag once; a solve that needs more continues in short replays.
+// It replaces the pipelined replays of `block` iterations, which queue two
+// full replays ahead of the first look and run up to 2 block - 1 no-op
+// iterations after the stop of a short solve.
 template <typename T>
 int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, int max_iter,
-            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code) {
+            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code, int first = 0) {
   double *part_patch, *part_node;
   int st;
   if ((st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)) ||
@@ -646,10 +652,22 @@ int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, i
   block = std::max(2, block + (block & 1));
   const int tb = tail_block_of(block);
   Replay exec, tail;
-  if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node, &exec)) ||
-      (st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)) ||
-      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb)))
+  if ((st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)))
     return st;
+  if (first > 0) {
+    first = std::max(2, first + (first & 1));
+    Replay one;
+    if ((st = pcg_graph<T>(ctx, w, sh, first, prec_tag, sc, part_patch, part_node, &one)))
+      return st;
+    HIPCHK(one.launch(ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (!ctx->h_sc->done && (st = replay_until(ctx, tail, sc, tb, max_iter))) return st;
+  } else if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node,
+                                &exec)) ||
+             (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb))) {
+    return st;
+  }
   femgpu::launch_pcg_apply_x<T>(ctx->dm, w.x, w.p, sc, ctx->stream);
   HIPCHK(hipGetLastError());
   *iters = ctx->h_sc->iter_final;
@@ -2356,6 +2374,10 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     CgScalars *sc;
     if ((st = dbuf(ctx, "sc_main", 1, &sc))) return st;
     int total_it = 0;
+    // iteration counts of the last two steps' solves: the next solve's first
+    // replay covers the larger (successive steps' solves take about the same
+    // number of iterations)
+    int it_prev[2] = {0, 0};
     int nt = 1;
     for (; nt <= endnt; ++nt) {
       femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, true, ctx->stream);
@@ -2371,9 +2393,11 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
                                     dv, w.r, C, ctx->stream, dml);
       int it = 0, done = 0;
       if ((st = run_pcg<T>(ctx, w, sh, FEMGPU_TOL_REF_GAMMA, o.pcg_eps, 1000000, 8,
-                           sizeof(T) == 8 ? 10 : 11, sc, &it, &done)))
+                           sizeof(T) == 8 ? 10 : 11, sc, &it, &done,
+                           std::max(it_prev[0], it_prev[1]))))
         return st;
       total_it += it;
+      it_prev[nt & 1] = it;
       if (done != 1) {
         S.status = done_status(done);
         ctx->err = "implicit step " + std::to_string(nt) + ": PCG stopped with " + std::to_string(done);
