@@ -628,46 +628,26 @@ int replay_until(femgpu_ctx *ctx, const Replay &exec, CgScalars *sc, int block, 
 inline int tail_block_of(int block) { return std::max(2, (block / 4) & ~1); }
 
 // Runs PCG until the stop flag is set.  r holds the initial residual; x and p
-// are cleared here.  first > 0 (a caller that knows about how many iterations
-// the solve takes, such as a time stepper from its previous steps): one
-// replay of `first` iterations (rounded up to even) and the host looks at
-// the stop flag once; a solve that needs more continues in short replays.
-// It replaces the pipelined replays of `block` iterations, which queue two
-// full replays ahead of the first look and run up to 2 block - 1 no-op
-// iterations after the stop of a short solve.
+// are cleared here.
 template <typename T>
 int run_pcg(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int kind, double tol, int max_iter,
-            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code, int first = 0) {
+            int block, int prec_tag, CgScalars *sc, int *iters, int *done_code) {
   double *part_patch, *part_node;
   int st;
   if ((st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)) ||
       (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
     return st;
   const int64_t nd = ctx->nn * ctx->dim;
-  femgpu::launch_fill<T>(w.x, nd, T(0), ctx->stream);
-  femgpu::launch_fill<T>(w.p, nd, T(0), ctx->stream);
-  femgpu::launch_reset_scalars(sc, kind, tol, max_iter, ctx->stream);
-  femgpu::launch_pcg_init<T>(ctx->dm, w.r, w.invD, sc, part_node, ctx->stream);
+  femgpu::launch_pcg_start<T>(ctx->dm, w.x, w.p, nd, w.r, w.invD, sc, kind, tol, max_iter,
+                              part_node, ctx->stream);
   HIPCHK(hipGetLastError());
   block = std::max(2, block + (block & 1));
   const int tb = tail_block_of(block);
   Replay exec, tail;
-  if ((st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)))
+  if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node, &exec)) ||
+      (st = pcg_graph<T>(ctx, w, sh, tb, prec_tag, sc, part_patch, part_node, &tail)) ||
+      (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb)))
     return st;
-  if (first > 0) {
-    first = std::max(2, first + (first & 1));
-    Replay one;
-    if ((st = pcg_graph<T>(ctx, w, sh, first, prec_tag, sc, part_patch, part_node, &one)))
-      return st;
-    HIPCHK(one.launch(ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (!ctx->h_sc->done && (st = replay_until(ctx, tail, sc, tb, max_iter))) return st;
-  } else if ((st = pcg_graph<T>(ctx, w, sh, block, prec_tag, sc, part_patch, part_node,
-                                &exec)) ||
-             (st = replay_until(ctx, exec, sc, block, max_iter, 0.0, &tail, tb))) {
-    return st;
-  }
   femgpu::launch_pcg_apply_x<T>(ctx->dm, w.x, w.p, sc, ctx->stream);
   HIPCHK(hipGetLastError());
   *iters = ctx->h_sc->iter_final;
@@ -2372,30 +2352,130 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     }
     w.invD = invA;
     CgScalars *sc;
-    if ((st = dbuf(ctx, "sc_main", 1, &sc))) return st;
-    int total_it = 0;
-    // iteration counts of the last two steps' solves: the next solve's first
-    // replay covers the larger (successive steps' solves take about the same
-    // number of iterations)
-    int it_prev[2] = {0, 0};
-    int nt = 1;
-    for (; nt <= endnt; ++nt) {
+    double *part_patch, *part_node;
+    if ((st = dbuf(ctx, "sc_main", 1, &sc)) ||
+        (st = dbuf(ctx, "part_patch", size_t(ctx->dm.np) + 1, &part_patch)) ||
+        (st = dbuf(ctx, "part_node", NODE_PART, &part_node)))
+      return st;
+    const int max_iter = 1000000;
+    const int prec_tag = sizeof(T) == 8 ? 10 : 11;
+    // The end of a step: x += alpha p still pending from the last update,
+    // a = x, then v and d, and the receivers.
+    auto step_end = [&]() {
+      femgpu::launch_pcg_apply_x<T>(ctx->dm, w.x, w.p, sc, ctx->stream);
+      femgpu::launch_copy<T>(w.x, da, int64_t(nd), ctx->stream);
+      femgpu::launch_dyn_implicit_post<T>(ctx->dm, da, dv, dd, C, ctx->stream);
+      record(0);
+    };
+    // The start of a step up to the PCG's first `first` iterations: the
+    // step counter (advanced on the device), predictor, K (d~ + beta v~),
+    // corrector into the PCG residual, PCG start.
+    auto step_begin = [&](int first) {
+      femgpu::launch_add_int(dstep, 1, ctx->stream);
       femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, true, ctx->stream);
       if (beta_d != 0.0) {
-        // K (d~ + beta v~)
         femgpu::launch_xpby<T>(int64_t(nd), dd, dv, beta_d, dt_tmp, ctx->stream);
         femgpu::launch_matvec_patch<T>(ctx->dm, dt_tmp, dy, dpart, ctx->stream);
       } else {
         femgpu::launch_matvec_patch<T>(ctx->dm, dd, dy, dpart, ctx->stream);
       }
-      femgpu::launch_set_int(dstep, nt, ctx->stream);
       femgpu::launch_dyn_correct<T>(ctx->dm, dy, dpart, dff, dfu, dcf, dcu, dstep, 0, nullptr, da,
                                     dv, w.r, C, ctx->stream, dml);
-      int it = 0, done = 0;
-      if ((st = run_pcg<T>(ctx, w, sh, FEMGPU_TOL_REF_GAMMA, o.pcg_eps, 1000000, 8,
-                           sizeof(T) == 8 ? 10 : 11, sc, &it, &done,
-                           std::max(it_prev[0], it_prev[1]))))
-        return st;
+      femgpu::launch_pcg_start<T>(ctx->dm, w.x, w.p, int64_t(nd), w.r, w.invD, sc,
+                                  FEMGPU_TOL_REF_GAMMA, o.pcg_eps, max_iter, part_node,
+                                  ctx->stream);
+      for (int k = 0; k < first; ++k)
+        femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.p, w.q, w.partial, w.invD, sh, sc,
+                                   part_patch, part_node, k & 1, ctx->stream);
+    };
+    // One launch per step: [end of the previous step,] start of this step
+    // and its first PCG replay, captured once per (with_end, first) and kept
+    // on the context under a key that names every pointer and value the
+    // launches hold (the step number is not among them).
+    std::string key_base;
+    {
+      char b[40];
+      const void *ptrs[] = {dd, dv, da, dy, dpart, dff, dfu, dcf, dcu, dstep, dml, dt_tmp,
+                            w.x, w.r, w.p, w.q, w.partial, w.invD, sc, part_patch, part_node,
+                            drecv, dtr, rsrc, sh.m};
+      const double vals[] = {C.dt, C.c1, C.c2, C.c3, C.c4, C.alpha, C.expl_scale, beta_d,
+                             double(sh.cK), double(sh.cM), o.pcg_eps,
+                             double(nrec), double(every), double(max_samples)};
+      key_base = "impl/" + std::to_string(prec_tag);
+      for (const void *q : ptrs) {
+        std::snprintf(b, sizeof(b), "/%p", q);
+        key_base += b;
+      }
+      for (double v : vals) {
+        std::snprintf(b, sizeof(b), "/%.17g", v);
+        key_base += b;
+      }
+    }
+    auto step_graph = [&](bool with_end, int first, Replay *out) -> int {
+      if (graphs_disabled()) {
+        out->exec = nullptr;
+        out->eager = [=, &step_end, &step_begin]() {
+          if (with_end) step_end();
+          step_begin(first);
+        };
+        return FEMGPU_OK;
+      }
+      if (ctx->graph_gen != ctx->buf_gen) {
+        drop_graphs(ctx);
+        ctx->graph_gen = ctx->buf_gen;
+      }
+      const std::string key =
+          key_base + "/" + std::to_string(int(with_end)) + "/" + std::to_string(first);
+      auto it = ctx->graphs.find(key);
+      if (it != ctx->graphs.end()) {
+        out->exec = it->second;
+        return FEMGPU_OK;
+      }
+      hipGraph_t g;
+      hipGraphExec_t exec = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      if (with_end) step_end();
+      step_begin(first);
+      HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+      const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(ie);
+      ctx->graphs[key] = exec;
+      out->exec = exec;
+      return FEMGPU_OK;
+    };
+    femgpu::launch_set_int(dstep, 0, ctx->stream);
+    int total_it = 0;
+    // The first replay covers the larger iteration count of the last two
+    // steps (successive steps' solves take about the same number), at most
+    // 64 iterations.  A solve that needs more continues, pipelined, in
+    // replays of two iterations when the count was foreseen (the stop is
+    // near), of 32 when the last steps took more than 64 (long solves, the
+    // stop prediction of replay_until ends them in short replays), of 8 on
+    // the first step (count unknown).
+    int it_prev[2] = {0, 0};
+    bool end_pending = false;
+    int nt = 1;
+    for (; nt <= endnt; ++nt) {
+      const int pred = std::max(it_prev[0], it_prev[1]);
+      int first = pred > 0 ? std::min(64, pred) : 8;
+      first = std::max(2, first + (first & 1));
+      const int cont_block = pred == 0 ? 8 : pred > first ? 32 : 2;
+      Replay step;
+      if ((st = step_graph(end_pending, first, &step))) return st;
+      HIPCHK(step.launch(ctx->stream));
+      HIPCHK(hipMemcpyAsync(ctx->h_sc, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      end_pending = false;
+      if (!ctx->h_sc->done) {
+        const int cb = cont_block, ctb = tail_block_of(cb);
+        Replay cont, tail;
+        if ((st = pcg_graph<T>(ctx, w, sh, cb, prec_tag, sc, part_patch, part_node, &cont)) ||
+            (st = pcg_graph<T>(ctx, w, sh, ctb, prec_tag, sc, part_patch, part_node, &tail)) ||
+            (st = replay_until(ctx, cont, sc, cb, max_iter, 0.0, &tail, ctb)))
+          return st;
+      }
+      const int it = ctx->h_sc->iter_final, done = ctx->h_sc->done;
       total_it += it;
       it_prev[nt & 1] = it;
       if (done != 1) {
@@ -2404,18 +2484,19 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
         if (stats) *stats = S;
         return S.status;
       }
-      femgpu::launch_copy<T>(w.x, da, int64_t(nd), ctx->stream);
-      femgpu::launch_dyn_implicit_post<T>(ctx->dm, da, dv, dd, C, ctx->stream);
-      record(0);
       if (relax) {
+        step_end();
         double vm = 0.0;
         if ((st = vmax(&vm))) return st;
         if (vm < eps_relax) {
           ++nt;
           break;
         }
+      } else {
+        end_pending = true;
       }
     }
+    if (end_pending) step_end();
     HIPCHK(hipGetLastError());
     steps = nt - 1;
     S.pcg_iterations = total_it;

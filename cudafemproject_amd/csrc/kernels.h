@@ -105,11 +105,14 @@ template <typename T>
 void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T *invD,
                      hipStream_t s);
 
-// PCG: init (gamma0 = r'z with z = invD r) and one fused iteration (z is
-// formed from r where it is needed, never stored).
+// PCG: start (x = p = 0 over n_all entries, every scalar reset as
+// launch_reset_scalars does, gamma0 = r'z with z = invD r) in two launches,
+// and one fused iteration (z is formed from r where it is needed, never
+// stored).
 template <typename T>
-void launch_pcg_init(const DevMesh &dm, const T *r, const T *invD, CgScalars *sc,
-                     double *part_node, hipStream_t s);
+void launch_pcg_start(const DevMesh &dm, T *x, T *p, int64_t n_all, const T *r, const T *invD,
+                      CgScalars *sc, int kind, double eps, int max_iter, double *part_node,
+                      hipStream_t s);
 // parity: j & 1 of the iteration (ping-pong of scalars and partial buffers).
 // part_patch: >= np doubles; part_node: 4 * NODE_PARTS doubles.
 template <typename T>
@@ -205,6 +208,8 @@ void launch_record(const int *step_base, int step_off, int every, const T *src, 
 template <typename T>
 void launch_absmax(int64_t n, const T *x, double *part, double *out, hipStream_t s);
 void launch_set_int(int *p, int v, hipStream_t s);
+// *p += v on the device (a step counter advanced inside a captured graph).
+void launch_add_int(int *p, int v, hipStream_t s);
 void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s);
 
 }  // namespace femgpu

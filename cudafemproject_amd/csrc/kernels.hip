@@ -639,13 +639,21 @@ __global__ void k_clear_x_pend(CgScalars *sc) {
 }
 
 // PCG initialisation: the r'z, r'r partials (z = invD r) that the first
-// fused iteration (parity 0) reads from slot 1.
+// fused iteration (parity 0) reads from slot 1; with x0 != nullptr also
+// x0 = p0 = 0 over n0 entries.
 template <typename T>
 __global__ __launch_bounds__(UPT) void k_pcg_init(int64_t n, const T *r, const T *invD,
-                                                  double *prz, double *prr) {
+                                                  double *prz, double *prr, T *x0, T *p0,
+                                                  int64_t n0) {
   __shared__ double s_red[UPT / 64];
   double rz = 0.0, rr = 0.0;
   const int64_t stride = int64_t(gridDim.x) * UPT;
+  if (x0) {
+    for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < n0; i += stride) {
+      x0[i] = T(0);
+      p0[i] = T(0);
+    }
+  }
   for (int64_t i = int64_t(blockIdx.x) * UPT + threadIdx.x; i < n; i += stride) {
     const T rv = r[i];
     const T zv = mul_rn(invD[i], rv);
@@ -660,10 +668,26 @@ __global__ __launch_bounds__(UPT) void k_pcg_init(int64_t n, const T *r, const T
   }
 }
 
+__device__ void reset_scalars(CgScalars *sc, int kind, double eps, int max_iter) {
+  sc->gamma0 = sc->rr0 = sc->thresh = 0.0;
+  sc->g[0] = sc->g[1] = 1.0;
+  sc->b[0] = sc->b[1] = 0.0;
+  sc->al[0] = sc->al[1] = 1.0;
+  sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = sc->pause = sc->x_pend = 0.0;
+  sc->it[0] = sc->it[1] = -1;
+  sc->iter_final = sc->par_final = 0;
+  sc->eps = eps;
+  sc->kind = kind;
+  sc->done = 0;
+  sc->max_iter = max_iter;
+}
+
 // Scalars of the initial residual (one block): gamma0, thresholds, and the
-// slot-1 values the first iteration reads (beta_0 is irrelevant: p = 0).
+// slot-1 values the first iteration reads (beta_0 is irrelevant: p = 0);
+// reset != 0 first resets every scalar as k_reset_scalars does.
 __global__ __launch_bounds__(NT) void k_fin_init(CgScalars *sc, const double *pa, const double *pb,
-                                                 int n) {
+                                                 int n, int reset, int kind, double eps,
+                                                 int max_iter) {
   __shared__ double s_red[NT / 64];
   double g = 0.0, q = 0.0;
   for (int i = threadIdx.x; i < n; i += NT) {
@@ -673,6 +697,7 @@ __global__ __launch_bounds__(NT) void k_fin_init(CgScalars *sc, const double *pa
   g = block_sum(g, s_red);
   q = block_sum(q, s_red);
   if (threadIdx.x == 0) {
+    if (reset) reset_scalars(sc, kind, eps, max_iter);
     sc->gamma0 = g;
     sc->rr0 = q;
     sc->g_final = g;
@@ -699,19 +724,7 @@ __global__ __launch_bounds__(NT) void k_fin_norm(CgScalars *sc, const double *pa
 }
 
 __global__ void k_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    sc->gamma0 = sc->rr0 = sc->thresh = 0.0;
-    sc->g[0] = sc->g[1] = 1.0;
-    sc->b[0] = sc->b[1] = 0.0;
-    sc->al[0] = sc->al[1] = 1.0;
-    sc->g_final = sc->rr_final = sc->pq = sc->rr = sc->scale = sc->pause = sc->x_pend = 0.0;
-    sc->it[0] = sc->it[1] = -1;
-    sc->iter_final = sc->par_final = 0;
-    sc->eps = eps;
-    sc->kind = kind;
-    sc->done = 0;
-    sc->max_iter = max_iter;
-  }
+  if (threadIdx.x == 0 && blockIdx.x == 0) reset_scalars(sc, kind, eps, max_iter);
 }
 
 // r = f - K x for the FP64 refinement loop; rr -> sc->rr, sqrt -> sc->scale.
@@ -1103,6 +1116,10 @@ __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+__global__ void k_add_int(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
 inline int grid_for(int64_t n) {
   int64_t g = (n + NT - 1) / NT;
   if (g < 1) g = 1;
@@ -1248,12 +1265,15 @@ void launch_inv_diag(const DevMesh &dm, const T *diag, T cK, T cM, const T *m, T
 }
 
 template <typename T>
-void launch_pcg_init(const DevMesh &dm, const T *r, const T *invD, CgScalars *sc,
-                     double *part_node, hipStream_t s) {
+void launch_pcg_start(const DevMesh &dm, T *x, T *p, int64_t n_all, const T *r, const T *invD,
+                      CgScalars *sc, int kind, double eps, int max_iter, double *part_node,
+                      hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
   double *prz = part_node + 2 * NODE_PARTS, *prr = part_node + 3 * NODE_PARTS;  // slot 1
-  hipLaunchKernelGGL((k_pcg_init<T>), dim3(NODE_PARTS), dim3(UPT), 0, s, n, r, invD, prz, prr);
-  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, prz, prr, NODE_PARTS);
+  hipLaunchKernelGGL((k_pcg_init<T>), dim3(NODE_PARTS), dim3(UPT), 0, s, n, r, invD, prz, prr, x,
+                     p, n_all);
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, prz, prr, NODE_PARTS, 1, kind, eps,
+                     max_iter);
 }
 
 template <typename T>
@@ -1333,7 +1353,7 @@ void launch_pcg_replace(const DevMesh &dm, const double *r64, float *r32, const 
 
 void launch_fin_init(CgScalars *sc, double *part_node, hipStream_t s) {
   hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, sc, part_node + 2 * NODE_PARTS,
-                     part_node + 3 * NODE_PARTS, NODE_PARTS);
+                     part_node + 3 * NODE_PARTS, NODE_PARTS, 0, 0, 0.0, 0);
 }
 
 void launch_pcg_resume(CgScalars *sc, hipStream_t s) {
@@ -1507,6 +1527,10 @@ void launch_set_int(int *p, int v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
 }
 
+void launch_add_int(int *p, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, s, p, v);
+}
+
 void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, kind, eps, max_iter);
 }
@@ -1517,8 +1541,8 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_diag<T>(const DevMesh &, T *, T *, hipStream_t);                        \
   template void launch_inv_diag<T>(const DevMesh &, const T *, T, T, const T *, T *,           \
                                    hipStream_t);                                               \
-  template void launch_pcg_init<T>(const DevMesh &, const T *, const T *, CgScalars *,         \
-                                   double *, hipStream_t);                                     \
+  template void launch_pcg_start<T>(const DevMesh &, T *, T *, int64_t, const T *, const T *,   \
+                                    CgScalars *, int, double, int, double *, hipStream_t);     \
   template void launch_pcg_apply_x<T>(const DevMesh &, T *, const T *, CgScalars *,            \
                                       hipStream_t);                                            \
   template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, const T *,        \
