@@ -2447,8 +2447,9 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     femgpu::launch_set_int(dstep, 0, ctx->stream);
     int total_it = 0;
     // The first replay covers the larger iteration count of the last two
-    // steps (successive steps' solves take about the same number), at most
-    // 64 iterations.  A solve that needs more continues, pipelined, in
+    // steps (successive steps' solves take about the same number) and the
+    // launch after it, whose stop test sees the converged residual (at most
+    // 64 launches).  A solve that needs more continues, pipelined, in
     // replays of two iterations when the count was foreseen (the stop is
     // near), of 32 when the last steps took more than 64 (long solves, the
     // stop prediction of replay_until ends them in short replays), of 8 on
@@ -2458,7 +2459,7 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     int nt = 1;
     for (; nt <= endnt; ++nt) {
       const int pred = std::max(it_prev[0], it_prev[1]);
-      int first = pred > 0 ? std::min(64, pred) : 8;
+      int first = pred > 0 ? std::min(64, pred + 1) : 8;
       first = std::max(2, first + (first & 1));
       const int cont_block = pred == 0 ? 8 : pred > first ? 32 : 2;
       Replay step;
