@@ -2277,12 +2277,13 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
   if (expl) {
     // Steps are captured in a HIP graph of G steps (G even, so every replay
     // starts on buffer 0) when at least one full graph fits the run; the
-    // remainder (or the whole run) is launched eagerly.
+    // remainder (or the whole run, and every step when graphs are disabled)
+    // is launched eagerly.
     int G = o.steps_per_graph > 0 ? o.steps_per_graph : 64;
     G += G & 1;
     int done = 0;
     bool settled = false;
-    if (G <= endnt) {
+    if (G <= endnt && !graphs_disabled()) {
       hipGraphExec_t exec = nullptr;
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
