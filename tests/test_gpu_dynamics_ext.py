@@ -221,6 +221,34 @@ def test_dynamic_relaxation_reaches_static_solution(ctx):
     assert rel(u, us) < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_implicit_relaxation_equals_fixed_run(ctx, precision):
+    """Implicit stepping with relaxation (max|v| checked after every step,
+    each step's end launched before the check) ends at the static solution
+    and equals, bitwise, a run of the same number of steps without the check
+    (each step's end captured with the next step's start)."""
+    pb = P.cube_problem(4, E=2e7, nu=0.25, pressure=1e3, convention=F.LOADS_PHYSICAL)
+    ctx.set_problem(pb)
+    us, _ = ctx.solve(F.FP64, tol=1e-12, tol_kind=F.TOL_REL_RES)
+    prec = F.FP64 if precision == "fp64" else F.FP32
+    kw = dict(rho=1000.0, beta1=0.5, beta2=0.5, dt=2e-4, mass_convention=F.MASS_PHYSICAL,
+              implicit_mode=F.IMPLICIT_PHYSICAL, pcg_eps=1e-10 if prec == F.FP64 else 1e-6,
+              damping_alpha=500.0, precision=prec)
+    # the distance to the static displacement is about max|v| at the stop
+    # over the slowest mode's decay rate (~40/s here): 1.4e-5 relative at
+    # max|v| < 1e-8, so 1e-10 (FP64) and 1e-7 (FP32)
+    eps = 1e-10 if prec == F.FP64 else 1e-7
+    u, st = ctx.dyn_run(endtime=-eps, relax_reference=False, max_steps=20000, **kw)
+    n = st["steps"]
+    print(f"implicit relaxation {precision}: {n} steps, {st['pcg_iterations']} PCG iterations, "
+          f"rel diff to static {rel(u, us):.2e}", flush=True)
+    assert st["status"] == 0 and 1 < n < 20000
+    assert rel(u, us) < (1e-5 if prec == F.FP64 else 1e-3)
+    u2, st2 = ctx.dyn_run(endtime=(n + 0.5) * kw["dt"], **kw)
+    assert st2["steps"] == n and st2["pcg_iterations"] == st["pcg_iterations"]
+    assert np.array_equal(u, u2)
+
+
 def test_relaxation_reference_stub_is_one_step(ctx):
     pb = column()
     ctx.set_problem(pb)
