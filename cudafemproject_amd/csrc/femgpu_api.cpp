@@ -2392,7 +2392,8 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     // One launch per step: [end of the previous step,] start of this step
     // and its first PCG replay, captured once per (with_end, first) and kept
     // on the context under a key that names every pointer and value the
-    // launches hold (the step number is not among them).
+    // launches hold besides the mesh and material, whose change drops every
+    // graph (the step number is not among them).
     std::string key_base;
     {
       char b[40];
