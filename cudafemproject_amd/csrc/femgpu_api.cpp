@@ -2360,20 +2360,23 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
       return st;
     const int max_iter = 1000000;
     const int prec_tag = sizeof(T) == 8 ? 10 : 11;
-    // The end of a step: x += alpha p still pending from the last update,
-    // a = x, then v and d, and the receivers.
+    // The PCG solves for the acceleration in place: its x is a (the
+    // corrector does not read a, the predictor reads it before the PCG
+    // start clears it).
+    w.x = da;
+    // The end of a step: x += alpha p still pending from the last update
+    // (the next PCG start resets the pending step), then v and d, and the
+    // receivers.
     auto step_end = [&]() {
-      femgpu::launch_pcg_apply_x<T>(ctx->dm, w.x, w.p, sc, ctx->stream);
-      femgpu::launch_copy<T>(w.x, da, int64_t(nd), ctx->stream);
+      femgpu::launch_pcg_apply_x<T>(ctx->dm, w.x, w.p, sc, ctx->stream, false);
       femgpu::launch_dyn_implicit_post<T>(ctx->dm, da, dv, dd, C, ctx->stream);
       record(0);
     };
     // The start of a step up to the PCG's first `first` iterations: the
-    // step counter (advanced on the device), predictor, K (d~ + beta v~),
-    // corrector into the PCG residual, PCG start.
+    // predictor, which also advances the device step counter, K (d~ + beta
+    // v~), corrector into the PCG residual, PCG start.
     auto step_begin = [&](int first) {
-      femgpu::launch_add_int(dstep, 1, ctx->stream);
-      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, true, ctx->stream);
+      femgpu::launch_dyn_predict<T>(ctx->dm, dd, dv, da, C, true, dstep, ctx->stream);
       if (beta_d != 0.0) {
         femgpu::launch_xpby<T>(int64_t(nd), dd, dv, beta_d, dt_tmp, ctx->stream);
         femgpu::launch_matvec_patch<T>(ctx->dm, dt_tmp, dy, dpart, ctx->stream);

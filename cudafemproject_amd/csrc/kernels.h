@@ -121,8 +121,11 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, cons
                      int parity, hipStream_t s);
 // After the last replay of a standard PCG run: the x += alpha p of its
 // last update, which the next iteration would have applied (sc->x_pend).
+// clear_pending = false leaves sc->x_pend set, for a caller whose next
+// PCG start resets the scalars anyway.
 template <typename T>
-void launch_pcg_apply_x(const DevMesh &dm, T *x, const T *p, CgScalars *sc, hipStream_t s);
+void launch_pcg_apply_x(const DevMesh &dm, T *x, const T *p, CgScalars *sc, hipStream_t s,
+                        bool clear_pending = true);
 constexpr int NODE_PARTS = 512;  // blocks of the node-parallel PCG kernels
 static_assert(NODE_PARTS % 256 == 0, "the patch kernels read the partials in chunks of 256");
 
@@ -162,9 +165,11 @@ struct DynCoeffs {
   double alpha = 0.0;         // Rayleigh mass coefficient (damping C = alpha M + beta K)
   double expl_scale = 1.0;    // explicit: 1 / (1 + alpha c3)
 };
+// Implicit predictor d += dt v + c1 a, v += c2 a (a = 0 with zero_a);
+// step_inc != nullptr: *step_inc += 1 (the device step counter) as well.
 template <typename T>
 void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
-                        hipStream_t s);
+                        int *step_inc, hipStream_t s);
 // r = F(step) - K d (interface finished from partials); explicit: a = r*invM, v += c3 a
 // (pred_v: v and a are the previous step's, the predicted v + c2 a is formed here).
 template <typename T>
@@ -208,8 +213,6 @@ void launch_record(const int *step_base, int step_off, int every, const T *src, 
 template <typename T>
 void launch_absmax(int64_t n, const T *x, double *part, double *out, hipStream_t s);
 void launch_set_int(int *p, int v, hipStream_t s);
-// *p += v on the device (a step counter advanced inside a captured graph).
-void launch_add_int(int *p, int v, hipStream_t s);
 void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s);
 
 }  // namespace femgpu

@@ -990,7 +990,8 @@ __global__ __launch_bounds__(NT) void k_bilinear(int64_t ne, const int32_t *elem
 
 template <typename T>
 __global__ __launch_bounds__(NT) void k_dyn_predict(int64_t n, T *d, T *v, T *a, T dt, T c1, T c2,
-                                                    int zero_a) {
+                                                    int zero_a, int *step) {
+  if (step && blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
   const int64_t stride = int64_t(gridDim.x) * NT;
   for (int64_t i = int64_t(blockIdx.x) * NT + threadIdx.x; i < n; i += stride) {
     const T av = a[i];
@@ -1114,10 +1115,6 @@ __global__ __launch_bounds__(NT) void k_absmax_fin(const double *part, int np, d
 
 __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
-}
-
-__global__ void k_add_int(int *p, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
 }
 
 inline int grid_for(int64_t n) {
@@ -1321,10 +1318,11 @@ void launch_pcg_iter(const DevMesh &dm, T *x, T *r, T *p, T *q, T *partial, cons
 }
 
 template <typename T>
-void launch_pcg_apply_x(const DevMesh &dm, T *x, const T *p, CgScalars *sc, hipStream_t s) {
+void launch_pcg_apply_x(const DevMesh &dm, T *x, const T *p, CgScalars *sc, hipStream_t s,
+                        bool clear_pending) {
   const int64_t n = dm.n_active * dm.dim;
   hipLaunchKernelGGL((k_pcg_apply_x<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, x, p, sc);
-  hipLaunchKernelGGL(k_clear_x_pend, dim3(1), dim3(64), 0, s, sc);
+  if (clear_pending) hipLaunchKernelGGL(k_clear_x_pend, dim3(1), dim3(64), 0, s, sc);
 }
 
 void launch_residual64(const DevMesh &dm, const double *f, const double *y, const double *partial,
@@ -1424,10 +1422,10 @@ void launch_bilinear(const DevMesh &dm, const T *u, const T *w, double factor, T
 
 template <typename T>
 void launch_dyn_predict(const DevMesh &dm, T *d, T *v, T *a, DynCoeffs c, bool zero_a,
-                        hipStream_t s) {
+                        int *step_inc, hipStream_t s) {
   const int64_t n = dm.n_active * dm.dim;
   hipLaunchKernelGGL((k_dyn_predict<T>), dim3(grid_for(n)), dim3(NT), 0, s, n, d, v, a, T(c.dt),
-                     T(c.c1), T(c.c2), int(zero_a));
+                     T(c.c1), T(c.c2), int(zero_a), step_inc);
 }
 
 template <typename T>
@@ -1527,10 +1525,6 @@ void launch_set_int(int *p, int v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
 }
 
-void launch_add_int(int *p, int v, hipStream_t s) {
-  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, s, p, v);
-}
-
 void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, sc, kind, eps, max_iter);
 }
@@ -1544,7 +1538,7 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_pcg_start<T>(const DevMesh &, T *, T *, int64_t, const T *, const T *,   \
                                     CgScalars *, int, double, int, double *, hipStream_t);     \
   template void launch_pcg_apply_x<T>(const DevMesh &, T *, const T *, CgScalars *,            \
-                                      hipStream_t);                                            \
+                                      hipStream_t, bool);                                      \
   template void launch_pcg_iter<T>(const DevMesh &, T *, T *, T *, T *, T *, const T *,        \
                                    OpShift<T>, CgScalars *, double *, double *, int,           \
                                    hipStream_t);                                               \
@@ -1553,7 +1547,7 @@ void launch_reset_scalars(CgScalars *sc, int kind, double eps, int max_iter, hip
   template void launch_stress<T>(const DevMesh &, const T *, T *, hipStream_t, bool);                \
   template void launch_energy<T>(const DevMesh &, const T *, T *, hipStream_t);                \
   template void launch_bilinear<T>(const DevMesh &, const T *, const T *, double, T *, hipStream_t); \
-  template void launch_dyn_predict<T>(const DevMesh &, T *, T *, T *, DynCoeffs, bool,         \
+  template void launch_dyn_predict<T>(const DevMesh &, T *, T *, T *, DynCoeffs, bool, int *,  \
                                       hipStream_t);                                            \
   template void launch_dyn_correct<T>(const DevMesh &, const T *, const T *, const T *,        \
                                       const T *, const double *, const double *, const int *,  \
