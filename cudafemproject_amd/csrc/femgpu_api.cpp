@@ -458,11 +458,12 @@ bool graphs_disabled() {
   return off;
 }
 
-// Graph of `block` fused PCG iterations (block even, so every replay starts
-// at parity 0) on these buffers; captured once and cached.
+// Graph of `block` fused PCG iterations on these buffers, the first at
+// parity p0 (block even, so every replay starts at the same parity);
+// captured once and cached.
 template <typename T>
 int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_tag,
-              CgScalars *sc, double *part_patch, double *part_node, Replay *out) {
+              CgScalars *sc, double *part_patch, double *part_node, Replay *out, int p0 = 0) {
   if (graphs_disabled()) {
     femgpu_ctx *c = ctx;
     PcgWs<T> ws = w;
@@ -470,7 +471,7 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
     out->eager = [=]() {
       for (int k = 0; k < block; ++k)
         femgpu::launch_pcg_iter<T>(c->dm, ws.x, ws.r, ws.p, ws.q, ws.partial, ws.invD, sh,
-                                   sc, part_patch, part_node, k & 1, c->stream);
+                                   sc, part_patch, part_node, (k + p0) & 1, c->stream);
     };
     return FEMGPU_OK;
   }
@@ -482,7 +483,7 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
   // (besides the mesh, whose change drops every graph)
   char keybuf[320];
   std::snprintf(keybuf, sizeof(keybuf),
-                "pcg/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block,
+                "pcg/%d/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%.17g/%.17g", prec_tag, block, p0,
                 (const void *)w.x, (const void *)w.r, (const void *)w.p,
                 (const void *)w.q, (const void *)w.partial, (const void *)w.invD, (const void *)sc,
                 (const void *)sh.m, (const void *)part_patch, (const void *)part_node,
@@ -498,7 +499,7 @@ int pcg_graph(femgpu_ctx *ctx, PcgWs<T> &w, OpShift<T> sh, int block, int prec_t
   HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < block; ++k)
     femgpu::launch_pcg_iter<T>(ctx->dm, w.x, w.r, w.p, w.q, w.partial, w.invD, sh, sc,
-                               part_patch, part_node, k & 1, ctx->stream);
+                               part_patch, part_node, (k + p0) & 1, ctx->stream);
   HIPCHK(hipStreamEndCapture(ctx->stream, &g));
   HIPCHK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
@@ -2458,14 +2459,14 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
     // replays of two iterations when the count was foreseen (the stop is
     // near), of 32 when the last steps took more than 64 (long solves, the
     // stop prediction of replay_until ends them in short replays), of 8 on
-    // the first step (count unknown).
+    // the first step (count unknown); after a first replay of odd length
+    // these start at parity 1.
     int it_prev[2] = {0, 0};
     bool end_pending = false;
     int nt = 1;
     for (; nt <= endnt; ++nt) {
       const int pred = std::max(it_prev[0], it_prev[1]);
-      int first = pred > 0 ? std::min(64, pred + 1) : 8;
-      first = std::max(2, first + (first & 1));
+      const int first = pred > 0 ? std::min(64, pred + 1) : 8;
       const int cont_block = pred == 0 ? 8 : pred > first ? 32 : 2;
       Replay step;
       if ((st = step_graph(end_pending, first, &step))) return st;
@@ -2476,8 +2477,9 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
       if (!ctx->h_sc->done) {
         const int cb = cont_block, ctb = tail_block_of(cb);
         Replay cont, tail;
-        if ((st = pcg_graph<T>(ctx, w, sh, cb, prec_tag, sc, part_patch, part_node, &cont)) ||
-            (st = pcg_graph<T>(ctx, w, sh, ctb, prec_tag, sc, part_patch, part_node, &tail)) ||
+        const int p0 = first & 1;
+        if ((st = pcg_graph<T>(ctx, w, sh, cb, prec_tag, sc, part_patch, part_node, &cont, p0)) ||
+            (st = pcg_graph<T>(ctx, w, sh, ctb, prec_tag, sc, part_patch, part_node, &tail, p0)) ||
             (st = replay_until(ctx, cont, sc, cb, max_iter, 0.0, &tail, ctb)))
           return st;
       }
