@@ -2417,6 +2417,27 @@ int dyn_run_t(femgpu_ctx *ctx, const femgpu_dyn_opts &o, double *u_out, femgpu_d
         key_base += b;
       }
     }
+    // Step graphs of other option sets (another dt, tolerance, damping)
+    // are released here once there are many: nothing launched from the
+    // cache is in flight or held between API calls.
+    {
+      const std::string own = key_base + "/";
+      auto stale = [&](const std::string &k) {
+        return k.compare(0, 5, "impl/") == 0 && k.compare(0, own.size(), own) != 0;
+      };
+      size_t n_stale = 0;
+      for (const auto &kv : ctx->graphs) n_stale += stale(kv.first);
+      if (n_stale > 256) {
+        for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();) {
+          if (stale(it->first)) {
+            (void)hipGraphExecDestroy(it->second);
+            it = ctx->graphs.erase(it);
+          } else {
+            ++it;
+          }
+        }
+      }
+    }
     auto step_graph = [&](bool with_end, int first, Replay *out) -> int {
       if (graphs_disabled()) {
         out->exec = nullptr;
