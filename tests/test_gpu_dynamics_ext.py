@@ -249,6 +249,34 @@ def test_implicit_relaxation_equals_fixed_run(ctx, precision):
     assert np.array_equal(u, u2)
 
 
+def test_implicit_step_graph_release(ctx):
+    """Implicit runs under many option sets (one dt each) leave their step
+    graphs cached until more than 256 belong to other option sets; the run
+    that releases them, and every run after, gives bitwise the results of the
+    same options in a fresh context."""
+    pb = column(8)
+    ctx.set_problem(pb)
+    kw = dict(KW, beta2=0.5, implicit_mode=F.IMPLICIT_PHYSICAL, pcg_eps=1e-8)
+
+    def run(c, dt):
+        u, st = c.dyn_run(**dict(kw, dt=dt, endtime=3.5 * dt))
+        assert st["steps"] == 3
+        return u
+
+    u_first = run(ctx, 4e-5)
+    for k in range(140):                         # 2 to 3 step graphs each
+        run(ctx, 4e-5 * (1.0 + 0.001 * (k + 1)))
+    u_again = run(ctx, 4e-5)
+    u_other = run(ctx, 4e-5 * 1.0005)
+    with F.Context(0) as c2:
+        c2.set_problem(pb)
+        u_ref = run(c2, 4e-5)
+        u_ref_other = run(c2, 4e-5 * 1.0005)
+    assert np.array_equal(u_first, u_ref)
+    assert np.array_equal(u_again, u_ref)
+    assert np.array_equal(u_other, u_ref_other)
+
+
 def test_relaxation_reference_stub_is_one_step(ctx):
     pb = column()
     ctx.set_problem(pb)
