@@ -331,7 +331,7 @@ class Context:
                 "gamma_ratio": s.gamma_ratio, "solve_ms": s.solve_ms}
 
     def solve(self, precision: int = FP64, tol: float = 1e-8, tol_kind: int = TOL_REF_GAMMA,
-              max_iter: int = 0, inner_tol: float = 0.1, check_every: int = 32,
+              max_iter: int = 0, inner_tol: float = 0.1, check_every: int = 64,
               raise_on_error: bool = True, recurrence: int = CG_STANDARD):
         o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every, recurrence)
         u = np.zeros((self.n_nodes, self.dim))
@@ -343,7 +343,7 @@ class Context:
 
     def solve_rhs(self, f: np.ndarray, precision: int = FP64, tol: float = 1e-8,
                   tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
-                  check_every: int = 32, recurrence: int = CG_STANDARD,
+                  check_every: int = 64, recurrence: int = CG_STANDARD,
                   raise_on_error: bool = True):
         """Solve K u = f for an explicit right-hand side (n_nodes x dim).
         raise_on_error=False returns the solution and stats of a solve that
@@ -367,7 +367,7 @@ class Context:
 
     def compliance_grad(self, precision: int = FP64, tol: float = 1e-8,
                         tol_kind: int = TOL_REL_RES, max_iter: int = 0, inner_tol: float = 0.1,
-                        check_every: int = 32, recurrence: int = CG_STANDARD):
+                        check_every: int = 64, recurrence: int = CG_STANDARD):
         """C = f'u and dC/ds_e from one solve; returns (C, g, u, stats)."""
         o = self._opts(precision, tol, tol_kind, max_iter, inner_tol, check_every, recurrence)
         u = np.zeros((self.n_nodes, self.dim))

@@ -1535,7 +1535,7 @@ int femgpu_default_solve_opts(femgpu_solve_opts *o) {
   o->tol = 1e-8;
   o->max_iter = 0;
   o->inner_tol = 0.1;
-  o->check_every = 32;
+  o->check_every = 64;
   o->recurrence = FEMGPU_CG_STANDARD;
   o->batch_schedule = FEMGPU_BATCH_LOCKSTEP;
   return FEMGPU_OK;
@@ -1712,7 +1712,7 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
   if (o.recurrence != FEMGPU_CG_STANDARD && o.recurrence != FEMGPU_CG_SINGLE_REDUCTION)
     return fail(ctx, FEMGPU_E_ARG, "unknown PCG recurrence");
   const int max_iter = o.max_iter > 0 ? o.max_iter : 1000000;
-  const int block = o.check_every > 0 ? o.check_every : 32;
+  const int block = o.check_every > 0 ? o.check_every : 64;
   if ((st = ensure_diag(ctx))) return st;
   const size_t nd = size_t(ctx->nn) * ctx->dim;
   double *f64, *x64;

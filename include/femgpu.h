@@ -224,7 +224,7 @@ typedef struct {
   double inner_tol;   /* MIXED: drop of the FP32 residual norm after which it
                          is replaced by the FP64 residual (default 0.1)       */
   int check_every;    /* iterations per graph replay between host polls of the
-                         stop flag (default 32); replays the host expects to
+                         stop flag (default 64); replays the host expects to
                          contain the stop run check_every / 4 (even, >= 2)    */
   int recurrence;     /* FEMGPU_CG_STANDARD (default) / _SINGLE_REDUCTION     */
   int batch_schedule; /* femgpu_batch_solve: FEMGPU_BATCH_LOCKSTEP (default) /
