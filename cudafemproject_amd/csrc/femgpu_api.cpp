@@ -597,8 +597,13 @@ int replay_until(femgpu_ctx *ctx, const Replay &exec, CgScalars *sc, int block, 
     return P.seen() && pause_host > 0.0 && P.r_last() * std::exp(P.lr * n) < pause_host;
   };
   // short replays from twice the horizon on: they cost a copy of the
-  // scalars each, a full replay past the stop up to block - 1 no-op launches
-  auto size_for = [&](int horizon) { return stop_within(2 * horizon) ? tail_block : block; };
+  // scalars each, a full replay past the stop up to block - 1 no-op launches;
+  // short too before the host has seen a replay of this solve (the first
+  // two are launched unseen), so that a short solve does not run two full
+  // replays of no-op launches
+  auto size_for = [&](int horizon) {
+    return !P.seen() || stop_within(2 * horizon) ? tail_block : block;
+  };
   if ((st = launch(0, size_for(block)))) return st;
   int cur = 0;
   bool queued = false;  // replay nxt was launched before looking at replay cur
