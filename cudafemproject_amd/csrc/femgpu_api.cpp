@@ -1783,7 +1783,10 @@ static int solve_core(femgpu_ctx *ctx, const femgpu_solve_opts *opts, const char
     HIPCHK(hipGetLastError());
     if ((st = poll(ctx, sc))) return st;
     const double ff = ctx->h_sc->rr0;
-    const int tb = tail_block_of(block_m);
+    // short replays of an eighth of a full one (a quarter elsewhere): each
+    // of the 4-5 drops to a residual replacement ends in them as well as the
+    // stop, and their no-op launches after the device pause add up
+    const int tb = std::max(2, (block_m / 8) & ~1);
     Replay exec, tail;
     if ((st = pcg_graph<float>(ctx, w, OpShift<float>(), block_m, 1, sc, part_patch, part_node,
                                &exec)) ||
